@@ -1,0 +1,249 @@
+"""Benchmark: % of HBM/MFMA roofline per routine on one MI355X.
+
+The headline line measures the sgemv configuration (y = A x, A FP32 row-major
+16384 x 8192, U[-1,1) from seed 1234): one step = one sgemv over the resident
+matrix.  `value` is the achieved fraction of the 8 TB/s HBM peak, in percent;
+`roofline` carries the absolute GB/s.  The `routines` object adds the other
+routines at their configured shapes (reduction n=1e6 and 5e8, sdot, native
+dgemm 4096^3, hgemm 4096^3, w4a16 M in {1,16,128,2048}).  `cpu_baseline`
+times the oracle's sgemv on the host (one thread).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK = 78.6      # TFLOP/s dense MFMA
+FP16_PEAK = 2500.0    # TFLOP/s dense MFMA
+M_SG, N_SG = 16384, 8192
+SG_BYTES = 4 * (M_SG * N_SG + N_SG + M_SG)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class Timer:
+    """HIP events on torch's current stream (the stream the kernels run on)."""
+
+    def __init__(self, torch):
+        self.torch = torch
+
+    def per_launch_ms(self, fn, iters, flush=None):
+        t = self.torch
+        if flush is None:
+            s, e = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+            s.record()
+            for _ in range(iters):
+                fn()
+            e.record()
+            e.synchronize()
+            return s.elapsed_time(e) / iters
+        total = 0.0
+        for _ in range(iters):
+            flush()
+            s, e = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+            s.record()
+            fn()
+            e.record()
+            e.synchronize()
+            total += s.elapsed_time(e)
+        return total / iters
+
+
+def traffic_from_profiles(kernel_key):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/traffic.json)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(kernel_key, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(seconds=10.0):
+    """The oracle's sgemv (sgemv.cpp:132-147 restated, one thread) on the host."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as orc
+    from libthom_amd import inputs
+    A = inputs.uniform_f32(M_SG * N_SG, -1.0, 1.0, 1234, inputs.S_SGEMV_A)
+    x = inputs.uniform_f32(N_SG, -1.0, 1.0, 1234, inputs.S_SGEMV_X)
+    y = np.zeros(M_SG, np.float32)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        orc.sgemv(True, M_SG, N_SG, 1.0, A, N_SG, x, 1, 0.0, y, 1)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    gbs = SG_BYTES * reps / el / 1e9
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"full 16384x8192 sgemv, {reps} runs in {el:.1f} s (oracle/oracle.c orc_sgemv, FP64 dot per row)"}
+
+
+def bench_routines(torch, R, timer, dev, quick):
+    import numpy as np
+    from libthom_amd import inputs
+    out = {}
+    flush_buf = torch.empty(512 * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+    flush_src = torch.empty_like(flush_buf)
+
+    def flush():
+        R.copy_f32(flush_buf, flush_src)
+
+    it = 5 if quick else 20
+    # reduction
+    for n in (1_000_000, 500_000_000):
+        x = torch.empty(n, dtype=torch.float32, device=dev)
+        R.fill_uniform(x, 0.0, 1.0, 1234, inputs.S_REDUCE)
+        o = torch.empty(1, dtype=torch.float32, device=dev)
+        R.reduce_sum(x, o)
+        ms = timer.per_launch_ms(lambda: R.reduce_sum(x, o), it, flush if n < 100_000_000 else None)
+        gbs = 4 * n / ms / 1e6
+        out[f"reduction_n{n}"] = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+                                  "GFLOP/s": round((n - 1) / ms / 1e6, 1)}
+        del x
+    # sdot
+    for n in (1_000_000, 250_000_000):
+        xs = torch.empty(n, dtype=torch.float32, device=dev)
+        ys = torch.empty(n, dtype=torch.float32, device=dev)
+        R.fill_uniform(xs, -1.0, 1.0, 1234, inputs.S_SDOT_X)
+        R.fill_uniform(ys, -1.0, 1.0, 1234, inputs.S_SDOT_Y)
+        o = torch.empty(1, dtype=torch.float32, device=dev)
+        R.sdot(xs, ys, o)
+        ms = timer.per_launch_ms(lambda: R.sdot(xs, ys, o), it, flush if n < 100_000_000 else None)
+        gbs = 8 * n / ms / 1e6
+        out[f"sdot_n{n}"] = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+        del xs, ys
+    torch.cuda.empty_cache()
+    # dgemm native
+    n = 4096
+    A = torch.from_numpy(inputs.normal_f64(n * n, 1234, inputs.S_DGEMM_A).reshape(n, n)).to(dev)
+    B = torch.from_numpy(inputs.normal_f64(n * n, 1234, inputs.S_DGEMM_B).reshape(n, n)).to(dev)
+    C = torch.empty(n, n, dtype=torch.float64, device=dev)
+    R.dgemm(A, B, C)
+    ms = timer.per_launch_ms(lambda: R.dgemm(A, B, C), max(3, it // 4))
+    tf = 2 * n ** 3 / ms / 1e9
+    out["dgemm_native_4096"] = {"ms": round(ms, 4), "TFLOP/s": round(tf, 2), "frac_fp64_mfma": round(tf / FP64_PEAK, 4)}
+    del A, B, C
+    # hgemm
+    for n in (4096, 8192):
+        A = torch.empty(n, n, dtype=torch.float16, device=dev)
+        B = torch.empty(n, n, dtype=torch.float16, device=dev)
+        R.fill_uniform(A, -1, 1, 1234, 31)
+        R.fill_uniform(B, -1, 1, 1234, 32)
+        C = torch.empty(n, n, dtype=torch.float16, device=dev)
+        R.hgemm(A, B, C)
+        ms = timer.per_launch_ms(lambda: R.hgemm(A, B, C), max(3, it // 4))
+        tf = 2 * n ** 3 / ms / 1e9
+        out[f"hgemm_{n}"] = {"ms": round(ms, 4), "TFLOP/s": round(tf, 1), "frac_fp16_mfma": round(tf / FP16_PEAK, 4),
+                             "config": repr(R.hgemm_default_config(n, n, n))}
+        del A, B, C
+    # w4a16
+    Nw, Kw, G = 11008, 4096, 128
+    for M in (1, 16, 128, 2048):
+        Aw, q, s, z = inputs.w4a16_problem(M, Nw, Kw, G)
+        At = torch.from_numpy(Aw).to(dev)
+        Wt = torch.from_numpy(inputs.pack_w4(q).view(np.int32)).to(dev)
+        St = torch.from_numpy(s).to(dev)
+        Zt = torch.from_numpy(inputs.pack_zeros(z).view(np.int32)).to(dev)
+        Ct = torch.empty(M, Nw, dtype=torch.float16, device=dev)
+        R.w4a16(At, Wt, St, Zt, G, Ct)
+        ms = timer.per_launch_ms(lambda: R.w4a16(At, Wt, St, Zt, G, Ct), it, flush)
+        byts = 2 * M * Kw + Kw // 8 * Nw * 4 + Kw // G * Nw * 2 + Kw // G * Nw // 2 + 2 * M * Nw
+        flops = 2 * M * Nw * Kw
+        out[f"w4a16_M{M}"] = {"us": round(ms * 1e3, 2), "GB/s": round(byts / ms / 1e6, 1),
+                              "frac_hbm": round(byts / ms / 1e6 / HBM_PEAK_GBS, 4),
+                              "TFLOP/s": round(flops / ms / 1e9, 2),
+                              "frac_fp16_mfma": round(flops / ms / 1e9 / FP16_PEAK, 4)}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--quick", action="store_true", help="fewer iterations for the per-routine table")
+    ap.add_argument("--no-routines", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md on sharding); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from libthom_amd import inputs, routines as R
+
+    dev = torch.device("cuda:0")
+    timer = Timer(torch)
+    log("allocating the sgemv workload (512 MiB matrix, filled on the GPU)")
+    A = torch.empty(M_SG, N_SG, dtype=torch.float32, device=dev)
+    R.fill_uniform(A, -1.0, 1.0, 1234, inputs.S_SGEMV_A)
+    x = torch.empty(N_SG, dtype=torch.float32, device=dev)
+    R.fill_uniform(x, -1.0, 1.0, 1234, inputs.S_SGEMV_X)
+    y = torch.zeros(M_SG, dtype=torch.float32, device=dev)
+    one = torch.ones(1, dtype=torch.float32, device=dev)
+    zero = torch.zeros(1, dtype=torch.float32, device=dev)
+
+    def step():
+        R.sgemv(A, x, y, alpha_dev=one, beta_dev=zero)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        step()
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / args.steps
+    gbs = SG_BYTES / ms / 1e6
+    log(f"sgemv 16384x8192: {ms * 1e3:.1f} us/step, {gbs:.0f} GB/s")
+
+    routines = None
+    if not args.no_routines:
+        del A
+        torch.cuda.empty_cache()
+        log("per-routine table")
+        routines = bench_routines(torch, R, timer, dev, args.quick)
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle sgemv, 1 thread)")
+        cpu = cpu_baseline(args.cpu_seconds)
+
+    line = {
+        "metric": "% of HBM/MFMA roofline per routine; emulated vs native DGEMM TFLOP/s @ equal err",
+        "value": round(100.0 * gbs / HBM_PEAK_GBS, 2),
+        "unit": "% of HBM peak (sgemv 16384x8192 FP32, 8 TB/s)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (SplitMix64 seed 1234, U[-1,1))",
+        "config": {"workload": "sgemv row-major y = A x, A 16384 x 8192 FP32", "M": M_SG, "N": N_SG,
+                   "bytes_per_step": SG_BYTES},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("sgemv_16384x8192")},
+        "cpu_baseline": cpu,
+        "routines": routines,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,148 @@
+/*
+ * lthip.h -- C ABI of the gfx950 (MI355X) implementations of libthom's routines.
+ *
+ * Conventions (every entry point):
+ *   - plain device pointers, sizes and scalars; `stream` is a hipStream_t passed
+ *     as void* (NULL = the null stream);
+ *   - returns LTHIP_OK (0) or an lthip_status; never exits, never prints;
+ *     lthip_last_error() gives a message for the calling thread;
+ *   - work is enqueued on `stream` and NOT synchronised: outputs are ready once
+ *     the stream is;
+ *   - entry points that need scratch take a caller-owned device workspace whose
+ *     size comes from the matching *_workspace_size() query.  Workspaces must be
+ *     zero-filled once (lthip_workspace_clear) before their first use; every call
+ *     leaves the counters they hold at zero again.  Calls that share one
+ *     workspace must not overlap in time.
+ *   - nothing here allocates device memory on the call path.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * thom-gg/libthom).  The reference's implementations are host functions
+ * returning elapsed milliseconds; the C++ adapters in libthom_amd/cli keep that
+ * signature and call these.
+ */
+#ifndef LTHIP_H
+#define LTHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum lthip_status {
+    LTHIP_OK = 0,
+    LTHIP_ERR_INVALID_ARG = 1,   /* bad size, stride, pointer or alignment */
+    LTHIP_ERR_UNSUPPORTED = 2,   /* valid request this build does not implement */
+    LTHIP_ERR_WORKSPACE = 3,     /* workspace missing or too small */
+    LTHIP_ERR_HIP = 4            /* the HIP runtime reported an error */
+};
+
+const char *lthip_status_string(int status);
+const char *lthip_last_error(void);
+/* Library version string (e.g. "0.1.0 gfx950"). */
+const char *lthip_version(void);
+
+/* Zero-fills `bytes` of a device workspace with a kernel (graph-capturable). */
+int lthip_workspace_clear(void *ws, size_t bytes, void *stream);
+
+/* ---- synthetic inputs (replace curandGenerateUniform, e.g. reduction.cpp:53,
+ *      sgemv.cpp:108-110, hgemm.cu:103-111) ----------------------------------
+ * Element i of (seed, stream_id) is x = SplitMix64 output i started at
+ * state seed + stream_id * 0xD1B54A32D192ED03; u = (x >> 40) * 2^-24 in [0,1);
+ * value = lo + (hi - lo) * u computed in double, then rounded to the output
+ * type (f16: double -> f32 -> f16, both round-to-nearest-even). */
+int lthip_fill_uniform_f32(float *out, size_t n, double lo, double hi, uint64_t seed,
+                           uint64_t stream_id, void *stream);
+int lthip_fill_uniform_f16(uint16_t *out, size_t n, double lo, double hi, uint64_t seed,
+                           uint64_t stream_id, void *stream);
+int lthip_fill_uniform_f64(double *out, size_t n, double lo, double hi, uint64_t seed,
+                           uint64_t stream_id, void *stream);
+
+/* Plain streaming copy dst[i] = src[i] (bandwidth probe and cache flush). */
+int lthip_copy_f32(float *dst, const float *src, size_t n, void *stream);
+
+/* ---- reduction: replaces ReductionImpl float(*)(const float*, float*, int)
+ *      (reduction.cpp:20; reduction_warp.cu:83).  *out = sum x[0..n) in FP32,
+ *      out is a device pointer.  Deterministic for a given n. -------------- */
+size_t lthip_reduce_sum_f32_workspace_size(size_t n);
+int lthip_reduce_sum_f32(const float *x, size_t n, float *out, void *ws, size_t ws_bytes,
+                         void *stream);
+
+/* ---- sdot: replaces SdotImpl float(*)(int, const float*, const float*, float*)
+ *      (sdot.cpp:32; sdot_warp.cu:101).  *result = sum x[i]*y[i] in FP32. -- */
+size_t lthip_sdot_f32_workspace_size(size_t n);
+int lthip_sdot_f32(size_t n, const float *x, const float *y, float *result, void *ws,
+                   size_t ws_bytes, void *stream);
+
+/* ---- sgemv: replaces SgemvImpl (sgemv.cpp:38; sgemv_vecto.cu:63,
+ *      sgemv_col3_vecto.cu:70).  y = alpha*op(A)x + beta*y, op(A) m x n,
+ *      stored row-major (row_major != 0, lda >= n) or column-major (lda >= m).
+ *      alpha and beta are DEVICE pointers, as in the reference (sgemv.cpp:112-118).
+ *      incx, incy > 0.  beta == 0 means y is not read.  Column-major needs
+ *      the workspace (partial sums); row-major ignores it. ---------------- */
+size_t lthip_sgemv_f32_workspace_size(int row_major, int m, int n);
+int lthip_sgemv_f32(int row_major, int m, int n, const float *alpha, const float *A, int lda,
+                    const float *x, int incx, const float *beta, float *y, int incy, void *ws,
+                    size_t ws_bytes, void *stream);
+
+/* ---- dgemm: replaces DgemmImpl (dgemm.cpp:36; dgemm_final.cu:175).  Row-major
+ *      A (M x K), B (K x N), C (M x N); C = alpha*A*B + beta*C on FP64 MFMA
+ *      (v_mfma_f64_16x16x4f64).  Any M, N, K >= 0.  beta == 0: C not read. --- */
+int lthip_dgemm_f64(const double *A, const double *B, double *C, size_t M, size_t N, size_t K,
+                    double alpha, double beta, void *stream);
+
+/* ---- hgemm: replaces HgemmFunction (hgemm.h:6-7; hgemm_better_reuse.cu:357).
+ *      Row-major A (M x K), B (K x N), C (M x N) fp16 (bit patterns);
+ *      C = fp16(alpha*A*B + beta*C) with FP32 accumulation on
+ *      v_mfma_f32_32x32x16_f16.  alpha/beta are fp16 bit patterns (the
+ *      reference passes __half by value).  K % 8 == 0, N % 8 == 0, A and B
+ *      16-byte aligned.  cfg NULL = built-in default; split_k > 1 needs the
+ *      workspace (FP32 partials + per-tile counters). ----------------------- */
+typedef struct lthip_hgemm_config {
+    int bm, bn, bk;   /* block tile; must be one of the built tiles */
+    int split_k;      /* >= 1 */
+    int stages;       /* LDS buffers: 2 (0 = default) */
+} lthip_hgemm_config;
+
+/* Enumerate the built tiles (the autotuner's search space; the reference's
+ * hgemm_better_reuse_tuning_candidates(), hgemm_better_reuse.cu:450-454). */
+int lthip_hgemm_config_count(void);
+int lthip_hgemm_config_get(int index, lthip_hgemm_config *cfg);
+/* The untuned choice (the reference's size-keyed default, hgemm_better_reuse.cu:357-376). */
+int lthip_hgemm_default_config(size_t M, size_t N, size_t K, lthip_hgemm_config *cfg);
+size_t lthip_hgemm_workspace_size(size_t M, size_t N, size_t K, const lthip_hgemm_config *cfg);
+int lthip_hgemm_f16(const uint16_t *A, const uint16_t *B, uint16_t *C, size_t M, size_t N, size_t K,
+                    uint16_t alpha, uint16_t beta, const lthip_hgemm_config *cfg, void *ws,
+                    size_t ws_bytes, void *stream);
+
+/* ---- w4a16: replaces W4A16Function float(*)(const W4A16Problem&) (w4a16.h:23-34;
+ *      w4a16_naive.cu:90) with per-(group, column) zero-points added.
+ *      A: M x K fp16 row-major.  W_packed: (K/8) x N uint32, row k of column n
+ *      in bits 4*(k%8) of W_packed[k/8][n] (quantize.cpp:42-53).
+ *      scales: (K/group) x N fp16.  qzeros: (K/group) x (N/8) uint32, z of
+ *      column n in bits 4*(n%8) of qzeros[g][n/8]; NULL means the reference's
+ *      fixed zero-point 8 (w4a16.h:19).  C: M x N fp16.
+ *      W[k][n] = fp16((q - z) * scale) exactly as quantize.cpp:58-65 (one
+ *      rounding); C = fp16(sum_k A[m][k] W[k][n]) with FP32 accumulation.
+ *      K % group == 0, group % 8 == 0, N % 8 == 0. ------------------------- */
+typedef struct lthip_w4a16_config {
+    int kind;        /* 0 = auto, 1 = small-M split-K stream, 2 = tiled MFMA */
+    int tile_n;      /* kind 1: 64 columns per block (fixed); kind 2: BN */
+    int tile_m;      /* kind 2: BM */
+    int split_k;     /* kind 1: K splits (>= 1) */
+} lthip_w4a16_config;
+
+size_t lthip_w4a16_workspace_size(size_t M, size_t N, size_t K, const lthip_w4a16_config *cfg);
+int lthip_w4a16_gemm(const uint16_t *A, const uint32_t *W_packed, const uint16_t *scales,
+                     const uint32_t *qzeros, uint16_t *C, size_t M, size_t N, size_t K, int group,
+                     const lthip_w4a16_config *cfg, void *ws, size_t ws_bytes, void *stream);
+/* Dequantise the whole weight matrix: W (K x N fp16) -- the bit-exact format
+ * check of dequantize_w4 (quantize.cpp:58-65) on the GPU. */
+int lthip_w4_dequant(const uint32_t *W_packed, const uint16_t *scales, const uint32_t *qzeros,
+                     uint16_t *W, size_t K, size_t N, int group, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

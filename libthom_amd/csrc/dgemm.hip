@@ -1,0 +1,159 @@
+// Native dgemm on FP64 MFMA (v_mfma_f64_16x16x4f64).
+//
+// Block tile 128x128x16, four waves in a 2x2 grid, each wave a 64x64 sub-tile
+// = 4x4 MFMA tiles (64 FP64 accumulators per lane).  The next K-slab is
+// fetched into registers while the current one is multiplied out of LDS; two
+// LDS buffers give one barrier per slab.  A is kept row-major in LDS with a
+// 17-double row stride (MFMA A-fragment reads: 16 lanes, 16 rows, conflict
+// free); B is kept K-major.  Tiles are visited in groups of 8 tile-rows and the
+// block ids are remapped so each XCD works on a contiguous run of tiles.
+//
+// Reference: dgemm_final.cu:65-213 (64x128x16 tiles, mma.m8n8k4, cp.async).
+#include "lt_common.hpp"
+
+namespace lt {
+namespace {
+
+constexpr int kBM = 128, kBN = 128, kBK = 16;
+constexpr int kAStride = kBK + 1;
+constexpr int kGroupM = 8;
+
+__device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int& tm, int& tn) {
+    const int per_group = kGroupM * tiles_n;
+    const int group = t / per_group;
+    const int first_m = group * kGroupM;
+    int gsize = tiles_m - first_m;
+    if (gsize > kGroupM) gsize = kGroupM;
+    const int r = t % per_group;
+    tm = first_m + r % gsize;
+    tn = r / gsize;
+}
+
+// FAST: M, N, K multiples of the tile and 16-byte aligned rows; otherwise
+// every element load/store is predicated.
+template <bool FAST>
+__global__ __launch_bounds__(256, 2) void dgemm_kernel(const double* __restrict__ A, const double* __restrict__ B,
+                                                       double* __restrict__ C, int M, int N, int K, double alpha,
+                                                       double beta, int tiles_m, int tiles_n) {
+    __shared__ double As[2][kBM][kAStride];
+    __shared__ double Bs[2][kBK][kBN];
+
+    int tm, tn;
+    tile_coords(xcd_remap(blockIdx.x, gridDim.x), tiles_m, tiles_n, tm, tn);
+    const int row0 = tm * kBM, col0 = tn * kBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+
+    // A slab: 128 rows x 16 k; thread -> row ar + 32 i, k pair akp
+    const int ar = tid >> 3, akp = (tid & 7) * 2;
+    // B slab: 16 k x 128 cols; thread -> k bk, cols bc + 2 i
+    const int bk = tid >> 4, bc = (tid & 15) * 8;
+
+    double2 ra[4], rb[4];
+
+    auto load_slab = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int r = row0 + ar + 32 * i, k = k0 + akp;
+            if (FAST) {
+                ra[i] = *reinterpret_cast<const double2*>(A + (size_t)r * K + k);
+            } else {
+                ra[i].x = (r < M && k < K) ? A[(size_t)r * K + k] : 0.0;
+                ra[i].y = (r < M && k + 1 < K) ? A[(size_t)r * K + k + 1] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int k = k0 + bk, c = col0 + bc + 2 * i;
+            if (FAST) {
+                rb[i] = *reinterpret_cast<const double2*>(B + (size_t)k * N + c);
+            } else {
+                rb[i].x = (k < K && c < N) ? B[(size_t)k * N + c] : 0.0;
+                rb[i].y = (k < K && c + 1 < N) ? B[(size_t)k * N + c + 1] : 0.0;
+            }
+        }
+    };
+    auto store_slab = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            As[buf][ar + 32 * i][akp] = ra[i].x;
+            As[buf][ar + 32 * i][akp + 1] = ra[i].y;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) *reinterpret_cast<double2*>(&Bs[buf][bk][bc + 2 * i]) = rb[i];
+    };
+
+    v4d acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    const int kslabs = (K + kBK - 1) / kBK;
+    if (kslabs > 0) {
+        load_slab(0);
+        store_slab(0);
+    }
+    __syncthreads();
+    const int fr = lane & 15, fk = lane >> 4;
+    for (int ks = 0; ks < kslabs; ks++) {
+        const int buf = ks & 1;
+        if (ks + 1 < kslabs) load_slab((ks + 1) * kBK);
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 4) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = As[buf][wm + 16 * i + fr][kk + fk];
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[j] = Bs[buf][kk + fk][wn + 16 * j + fr];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (ks + 1 < kslabs) store_slab(buf ^ 1);
+        __syncthreads();
+    }
+
+    // C fragment of a 16x16 f64 tile: register r of lane l holds row 4*r + l/16,
+    // column l%16 (measured; differs from the f32 16x16 layout)
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = row0 + wm + 16 * i + 4 * r + fk, col = col0 + wn + 16 * j + fr;
+                if (FAST || (row < M && col < N)) {
+                    double* cp = C + (size_t)row * N + col;
+                    double v = alpha * acc[i][j][r];
+                    if (beta != 0.0) v = fma(beta, *cp, v);
+                    *cp = v;
+                }
+            }
+}
+
+}  // namespace
+}  // namespace lt
+
+extern "C" int lthip_dgemm_f64(const double* A, const double* B, double* C, size_t M, size_t N, size_t K,
+                               double alpha, double beta, void* stream) {
+    using namespace lt;
+    if (M > (1u << 30) || N > (1u << 30) || K > (1u << 30)) return fail(LTHIP_ERR_INVALID_ARG, "dgemm: size too large");
+    if (M == 0 || N == 0) return LTHIP_OK;
+    if (!C || (K > 0 && (!A || !B))) return fail(LTHIP_ERR_INVALID_ARG, "dgemm: null pointer");
+    const int m = (int)M, n = (int)N, k = (int)K;
+    const int tiles_m = (m + kBM - 1) / kBM, tiles_n = (n + kBN - 1) / kBN;
+    const long blocks = (long)tiles_m * tiles_n;
+    if (blocks > 0x7fffffffL) return fail(LTHIP_ERR_INVALID_ARG, "dgemm: too many tiles");
+    const bool fast = (m % kBM == 0) && (n % kBN == 0) && (k % kBK == 0) && (reinterpret_cast<uintptr_t>(A) % 16 == 0) &&
+                      (reinterpret_cast<uintptr_t>(B) % 16 == 0);
+    if (fast)
+        hipLaunchKernelGGL((dgemm_kernel<true>), dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), A, B, C, m, n,
+                           k, alpha, beta, tiles_m, tiles_n);
+    else
+        hipLaunchKernelGGL((dgemm_kernel<false>), dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), A, B, C, m,
+                           n, k, alpha, beta, tiles_m, tiles_n);
+    LT_TRY_LAUNCH("dgemm launch");
+    return LTHIP_OK;
+}

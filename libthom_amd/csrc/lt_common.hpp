@@ -1,0 +1,111 @@
+// Shared device helpers and host plumbing for the gfx950 kernels behind lthip.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "lthip.h"
+
+namespace lt {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int kWave = 64;
+
+// Split-K workspaces: a fixed head of arrival counters (zero between calls),
+// then the FP32 partials.  The head never holds partials, so a later call with
+// more tiles still finds zeroed counters.
+constexpr size_t kCounterBytes = 65536;
+constexpr size_t kMaxCounters = kCounterBytes / sizeof(unsigned);
+
+// ---- host side --------------------------------------------------------
+// Records a message for lthip_last_error() and returns the status.
+int fail(int status, const char* fmt, ...);
+// Maps a HIP error to LTHIP_ERR_HIP (recording the HIP message) or LTHIP_OK.
+int check_hip(hipError_t e, const char* what);
+// Compute units of the current device (cached per device id).
+int cu_count();
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+#define LT_TRY_LAUNCH(what)                                  \
+    do {                                                     \
+        int st__ = ::lt::check_hip(hipGetLastError(), what); \
+        if (st__ != LTHIP_OK) return st__;                   \
+    } while (0)
+
+// ---- device side ------------------------------------------------------
+// Streaming (non-temporal) 16-byte loads/stores: the data is used once.
+__device__ __forceinline__ float4 ld_nt(const float4* p) {
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
+    return make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ uint4 ld_nt(const uint4* p) {
+    const v4i v = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(p));
+    return make_uint4((unsigned)v[0], (unsigned)v[1], (unsigned)v[2], (unsigned)v[3]);
+}
+__device__ __forceinline__ void st_nt(float4* p, float4 a) {
+    const v4f v = {a.x, a.y, a.z, a.w};
+    __builtin_nontemporal_store(v, reinterpret_cast<v4f*>(p));
+}
+
+// DPP move within rows of 16 lanes; every control used here reads a valid lane.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float readlane_f32(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Sum over the 64 lanes of a wave, returned to every lane.  Rows of 16 are
+// reduced with DPP (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror,
+// row_mirror), then the four row sums are combined in a fixed order, so the
+// result does not depend on scheduling.
+__device__ __forceinline__ float wave_sum(float v) {
+    v += dpp_f32<0xB1>(v);
+    v += dpp_f32<0x4E>(v);
+    v += dpp_f32<0x141>(v);
+    v += dpp_f32<0x140>(v);
+    return (readlane_f32(v, 0) + readlane_f32(v, 16)) + (readlane_f32(v, 32) + readlane_f32(v, 48));
+}
+
+// Block sum for BLOCK threads (a multiple of 64); valid in every thread.
+// `scratch` must hold BLOCK/64 floats of shared memory.
+template <int BLOCK>
+__device__ __forceinline__ float block_sum(float v, float* scratch) {
+    constexpr int W = BLOCK / kWave;
+    v = wave_sum(v);
+    if (W == 1) return v;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    float s = 0.0f;
+#pragma unroll
+    for (int w = 0; w < W; w++) s += scratch[w];
+    __syncthreads();
+    return s;
+}
+
+// Remap a linear block id so that the blocks the hardware sends to one XCD
+// (ids congruent mod 8) cover a contiguous range of logical tiles: neighbouring
+// tiles then share operand panels in the same L2.  Ids past the last full
+// group of 8 are left unchanged.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+    constexpr int kXcd = 8;
+    const int full = (nblocks / kXcd) * kXcd;
+    if (bid >= full) return bid;
+    const int per = full / kXcd;
+    return (bid % kXcd) * per + bid / kXcd;
+}
+
+}  // namespace lt

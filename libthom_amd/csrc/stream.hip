@@ -1,0 +1,225 @@
+// HBM-streaming kernels: input generation, copy, reduction and sdot.
+//
+// reduction / sdot are one launch each: a grid-stride float4 stream with four
+// independent loads in flight per lane, a DPP wave sum, an LDS block sum, and a
+// last-block-done finish (the block that takes the final ticket of an agent-
+// scope counter sums the per-block partials in block order).  The reference
+// needs two launches plus a cudaMalloc inside its timed window
+// (reduction_warp.cu:94-105, sdot_warp.cu:117-136).
+#include "lt_common.hpp"
+
+namespace lt {
+namespace {
+
+constexpr int kRBlock = 256;
+constexpr int kRUnroll = 4;
+constexpr unsigned kRMaxBlocks = 4096;
+constexpr size_t kRWorkspace = 256 + sizeof(float) * kRMaxBlocks;   // counter + partials
+
+// ---- SplitMix64 fill ---------------------------------------------------
+__device__ __forceinline__ uint64_t sm_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double uniform_at(uint64_t base, size_t i, double lo, double span) {
+    const uint64_t x = sm_mix(base + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull);
+    const double u = (double)(x >> 40) * 0x1p-24;
+    return __dadd_rn(lo, __dmul_rn(span, u));   // no contraction: matches the host spec
+}
+
+template <typename T>
+__device__ __forceinline__ T from_double(double v);
+template <>
+__device__ __forceinline__ float from_double<float>(double v) { return (float)v; }
+template <>
+__device__ __forceinline__ double from_double<double>(double v) { return v; }
+template <>
+__device__ __forceinline__ uint16_t from_double<uint16_t>(double v) {
+    return __half_as_ushort(__float2half_rn((float)v));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void fill_uniform_kernel(T* __restrict__ out, size_t n, double lo, double span,
+                                                           uint64_t base) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = from_double<T>(uniform_at(base, i, lo, span));
+}
+
+template <typename T>
+int fill_uniform(T* out, size_t n, double lo, double hi, uint64_t seed, uint64_t stream_id, void* stream) {
+    if (n == 0) return LTHIP_OK;
+    if (!out) return fail(LTHIP_ERR_INVALID_ARG, "fill_uniform: null output");
+    const uint64_t base = seed + stream_id * 0xD1B54A32D192ED03ull;
+    size_t blocks = (n + 255) / 256;
+    const size_t cap = (size_t)cu_count() * 16;
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(fill_uniform_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), out, n, lo,
+                       hi - lo, base);
+    LT_TRY_LAUNCH("fill_uniform launch");
+    return LTHIP_OK;
+}
+
+// ---- copy ---------------------------------------------------------------
+__global__ __launch_bounds__(256) void copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src, size_t n4,
+                                                   float* __restrict__ dtail, const float* __restrict__ stail,
+                                                   size_t ntail) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t i = tid;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        float4 a = ld_nt(&src[i]);
+        float4 b = ld_nt(&src[i + stride]);
+        float4 c = ld_nt(&src[i + 2 * stride]);
+        float4 d = ld_nt(&src[i + 3 * stride]);
+        st_nt(&dst[i], a);
+        st_nt(&dst[i + stride], b);
+        st_nt(&dst[i + 2 * stride], c);
+        st_nt(&dst[i + 3 * stride], d);
+    }
+    for (; i < n4; i += stride) dst[i] = src[i];
+    if (tid < ntail) dtail[tid] = stail[tid];
+}
+
+// ---- reduction / dot ---------------------------------------------------
+template <bool DOT>
+__device__ __forceinline__ float term4(float4 a, float4 b) {
+    if (DOT) return fmaf(a.x, b.x, a.y * b.y) + fmaf(a.z, b.z, a.w * b.w);
+    return (a.x + a.y) + (a.z + a.w);
+}
+
+// VEC: x (and y) 16-byte aligned, float4 path with a scalar tail.
+template <bool DOT, bool VEC>
+__global__ __launch_bounds__(kRBlock) void sum_kernel(const float* __restrict__ x, const float* __restrict__ y, size_t n,
+                                                      float* __restrict__ out, unsigned* __restrict__ counter,
+                                                      float* __restrict__ partials) {
+    __shared__ float red[kRBlock / kWave];
+    __shared__ int is_last;
+    const size_t nthreads = (size_t)gridDim.x * kRBlock;
+    const size_t tid = (size_t)blockIdx.x * kRBlock + threadIdx.x;
+    float acc[kRUnroll] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (VEC) {
+        const size_t n4 = n >> 2;
+        const float4* x4 = reinterpret_cast<const float4*>(x);
+        const float4* y4 = reinterpret_cast<const float4*>(y);
+        size_t i = tid;
+        for (; i + (kRUnroll - 1) * nthreads < n4; i += kRUnroll * nthreads) {
+            float4 a[kRUnroll], b[kRUnroll];
+#pragma unroll
+            for (int u = 0; u < kRUnroll; u++) {
+                a[u] = ld_nt(&x4[i + u * nthreads]);
+                if (DOT) b[u] = ld_nt(&y4[i + u * nthreads]);
+            }
+#pragma unroll
+            for (int u = 0; u < kRUnroll; u++) acc[u] += term4<DOT>(a[u], DOT ? b[u] : a[u]);
+        }
+        for (; i < n4; i += nthreads) {
+            float4 a = x4[i];
+            acc[0] += term4<DOT>(a, DOT ? y4[i] : a);
+        }
+        const size_t t = 4 * n4 + tid;   // n % 4 leftover elements, one per thread
+        if (t < n) acc[1] += DOT ? x[t] * y[t] : x[t];
+    } else {
+        for (size_t i = tid; i < n; i += nthreads) acc[0] += DOT ? x[i] * y[i] : x[i];
+    }
+    float s = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    s = block_sum<kRBlock>(s, red);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[blockIdx.x], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned ticket = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (ticket == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    float p = 0.0f;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += kRBlock)
+        p += __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    p = block_sum<kRBlock>(p, red);
+    if (threadIdx.x == 0) {
+        *out = p;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+unsigned sum_blocks(size_t n) {
+    const size_t per_block = (size_t)kRBlock * 4 * kRUnroll;   // floats one block covers in one unrolled step
+    size_t b = (n + per_block - 1) / per_block;
+    const size_t cap = (size_t)cu_count() * 8;
+    if (b > cap) b = cap;
+    if (b > kRMaxBlocks) b = kRMaxBlocks;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+template <bool DOT>
+int launch_sum(const float* x, const float* y, size_t n, float* out, void* ws, size_t ws_bytes, void* stream) {
+    if (!out) return fail(LTHIP_ERR_INVALID_ARG, "null output pointer");
+    if (n > 0 && (!x || (DOT && !y))) return fail(LTHIP_ERR_INVALID_ARG, "null input pointer");
+    if (!ws || ws_bytes < kRWorkspace) return fail(LTHIP_ERR_WORKSPACE, "need %zu workspace bytes", kRWorkspace);
+    if (reinterpret_cast<uintptr_t>(ws) % 16 != 0)
+        return fail(LTHIP_ERR_INVALID_ARG, "workspace must be 16-byte aligned");
+    unsigned* counter = reinterpret_cast<unsigned*>(ws);
+    float* partials = reinterpret_cast<float*>(reinterpret_cast<char*>(ws) + 256);
+    const unsigned blocks = sum_blocks(n);
+    const bool vec = (reinterpret_cast<uintptr_t>(x) % 16 == 0) && (!DOT || reinterpret_cast<uintptr_t>(y) % 16 == 0);
+    if (vec)
+        hipLaunchKernelGGL((sum_kernel<DOT, true>), dim3(blocks), dim3(kRBlock), 0, as_stream(stream), x, y, n, out,
+                           counter, partials);
+    else
+        hipLaunchKernelGGL((sum_kernel<DOT, false>), dim3(blocks), dim3(kRBlock), 0, as_stream(stream), x, y, n, out,
+                           counter, partials);
+    LT_TRY_LAUNCH(DOT ? "sdot launch" : "reduction launch");
+    return LTHIP_OK;
+}
+
+}  // namespace
+}  // namespace lt
+
+extern "C" {
+
+int lthip_fill_uniform_f32(float* out, size_t n, double lo, double hi, uint64_t seed, uint64_t stream_id,
+                           void* stream) {
+    return lt::fill_uniform<float>(out, n, lo, hi, seed, stream_id, stream);
+}
+int lthip_fill_uniform_f16(uint16_t* out, size_t n, double lo, double hi, uint64_t seed, uint64_t stream_id,
+                           void* stream) {
+    return lt::fill_uniform<uint16_t>(out, n, lo, hi, seed, stream_id, stream);
+}
+int lthip_fill_uniform_f64(double* out, size_t n, double lo, double hi, uint64_t seed, uint64_t stream_id,
+                           void* stream) {
+    return lt::fill_uniform<double>(out, n, lo, hi, seed, stream_id, stream);
+}
+
+int lthip_copy_f32(float* dst, const float* src, size_t n, void* stream) {
+    if (n == 0) return LTHIP_OK;
+    if (!dst || !src) return lt::fail(LTHIP_ERR_INVALID_ARG, "copy: null pointer");
+    if (reinterpret_cast<uintptr_t>(dst) % 16 || reinterpret_cast<uintptr_t>(src) % 16)
+        return lt::fail(LTHIP_ERR_INVALID_ARG, "copy: pointers must be 16-byte aligned");
+    const size_t n4 = n / 4, ntail = n % 4;
+    size_t blocks = (n4 + 1023) / 1024;
+    const size_t cap = (size_t)lt::cu_count() * 8;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(lt::copy_kernel, dim3((unsigned)blocks), dim3(256), 0, lt::as_stream(stream),
+                       reinterpret_cast<float4*>(dst), reinterpret_cast<const float4*>(src), n4, dst + 4 * n4,
+                       src + 4 * n4, ntail);
+    LT_TRY_LAUNCH("copy launch");
+    return LTHIP_OK;
+}
+
+size_t lthip_reduce_sum_f32_workspace_size(size_t) { return lt::kRWorkspace; }
+size_t lthip_sdot_f32_workspace_size(size_t) { return lt::kRWorkspace; }
+
+int lthip_reduce_sum_f32(const float* x, size_t n, float* out, void* ws, size_t ws_bytes, void* stream) {
+    return lt::launch_sum<false>(x, nullptr, n, out, ws, ws_bytes, stream);
+}
+
+int lthip_sdot_f32(size_t n, const float* x, const float* y, float* result, void* ws, size_t ws_bytes,
+                   void* stream) {
+    return lt::launch_sum<true>(x, y, n, result, ws, ws_bytes, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,335 @@
+// w4a16: C (M x N fp16) = A (M x K fp16) * dequant(W int4), FP32 accumulation.
+//
+// Weight format (quantize.cpp:42-65, generalised with zero-points):
+//   W_packed[k/8][n] bits 4*(k%8) = q[k][n]; scales[k/group][n] fp16;
+//   qzeros[k/group][n/8] bits 4*(n%8) = z (null: z = 8);
+//   W[k][n] = fp16((q - z) * scale), rounded once.
+//
+// Because 8 consecutive k of one column live in one uint32, a dequantised
+// uint32 is exactly one MFMA operand fragment (8 halves along K).
+//
+// kind 1, small M (decode, M <= 64): MFMA 16x16x32 on C^T = W^T A^T.  A wave
+// owns 64 columns; lane l loads the uint4 W[kp][n0 + 4(l%16) .. +3] (kp = k-block
+// l/16), so MFMA tile j covers columns n0 + 4r + j.  The activations are the
+// B operand (16 rows of A per MFMA).  The four waves of a block split the
+// block's K range; K is further split across blocks, whose FP32 partials are
+// summed in split order by the last block to finish (deterministic).
+//
+// kind 2, large M: the shared tiled core (gemm_core.hpp) with the int4 loader,
+// which dequantises each K-slab of W into LDS once per block.
+#include "gemm_core.hpp"
+
+namespace lt {
+namespace {
+
+__global__ __launch_bounds__(256) void dequant_kernel(const uint32_t* __restrict__ W, const __half* __restrict__ scales,
+                                                      const uint32_t* __restrict__ qz, __half* __restrict__ out, int K,
+                                                      int N, int group) {
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    const long total = (long)(K / 8) * N;
+    if (idx >= total) return;
+    const int kp = (int)(idx / N), n = (int)(idx % N);
+    const uint32_t w = W[(long)kp * N + n];
+    const int g = kp * 8 / group;
+    const uint32_t z = qz ? (qz[(long)g * (N / 8) + n / 8] >> (4 * (n & 7))) & 0xFu : 8u;
+    const uint32_t s = __half_as_ushort(scales[(long)g * N + n]);
+    const uint32_t zz = (0x6400u + z) * 0x10001u;
+    const uint4 d = dequant8(w, zz, s | (s << 16));   // k order [0,4,1,5,2,6,3,7]
+    const uint32_t dv[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        out[(long)(8 * kp + j) * N + n] = __ushort_as_half((unsigned short)(dv[j] & 0xFFFFu));
+        out[(long)(8 * kp + j + 4) * N + n] = __ushort_as_half((unsigned short)(dv[j] >> 16));
+    }
+}
+
+constexpr int kStreamCols = 64;
+
+template <int MT>
+__global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restrict__ A, BW4Params p,
+                                                           __half* __restrict__ C, int M, int N, int K, int splits,
+                                                           float* __restrict__ partial, unsigned* __restrict__ counters) {
+    __shared__ float red[3][MT * 16][64];
+    __shared__ int last_flag;
+    const int ntile = blockIdx.x, split = blockIdx.y;
+    const int n0 = ntile * kStreamCols;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, kb = lane >> 4;
+    const int ncol = n0 + 4 * r;
+    const bool colv = ncol < N;
+    const int KP = K / 8;
+    const int ksteps = (KP + 3) / 4;   // 32 k per MFMA step
+    const int ks0 = (int)((long)ksteps * split / splits), ks1 = (int)((long)ksteps * (split + 1) / splits);
+
+    v4f acc[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[mt][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    for (int ks = ks0 + wave; ks < ks1; ks += 4) {
+        const int kp = ks * 4 + kb;
+        const bool kv = kp < KP;
+        uint4 w = make_uint4(0, 0, 0, 0);
+        uint2 s = make_uint2(0, 0);
+        uint32_t z = 0;
+        if (kv && colv) {
+            const int g = kp * 8 / p.group;
+            w = ld_nt(reinterpret_cast<const uint4*>(p.W + (long)kp * N + ncol));
+            s = *reinterpret_cast<const uint2*>(p.scales + (long)g * N + ncol);
+            z = p.qzeros ? (p.qzeros[(long)g * (N / 8) + ncol / 8] >> (4 * (ncol & 7))) & 0xFFFFu : 0x8888u;
+        }
+        const uint32_t wv[4] = {w.x, w.y, w.z, w.w};
+        const uint32_t sv[4] = {s.x & 0xFFFFu, s.x >> 16, s.y & 0xFFFFu, s.y >> 16};
+        v8h wf[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t zj = 0x6400u + ((z >> (4 * j)) & 0xFu);
+            const uint4 d = dequant8(wv[j], zj | (zj << 16), sv[j] | (sv[j] << 16));
+            __builtin_memcpy(&wf[j], &d, 16);
+        }
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) {
+            const int m = 16 * mt + r;
+            uint4 a = make_uint4(0, 0, 0, 0);
+            if (kv && m < M) a = *reinterpret_cast<const uint4*>(A + (long)m * K + 8 * kp);
+            a = perm_k8(a);
+            v8h af;
+            __builtin_memcpy(&af, &a, 16);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[mt][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wf[j], af, acc[mt][j], 0, 0, 0);
+        }
+    }
+
+    // cross-wave sum in wave order (deterministic): waves 1..3 park, wave 0 adds
+    if (wave > 0) {
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) red[wave - 1][mt * 16 + j * 4 + i][lane] = acc[mt][j][i];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    float v = acc[mt][j][i];
+                    v += red[0][mt * 16 + j * 4 + i][lane];
+                    v += red[1][mt * 16 + j * 4 + i][lane];
+                    v += red[2][mt * 16 + j * 4 + i][lane];
+                    acc[mt][j][i] = v;
+                }
+    }
+    // acc[mt][j][i] of lane l is C[m = 16 mt + l%16][n = n0 + 16 (l/16) + 4 i + j]
+    if (splits <= 1) {
+        if (wave != 0) return;
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) {
+            const int m = 16 * mt + r;
+            if (m >= M) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int n = n0 + 16 * kb + 4 * i;
+                if (n >= N) continue;
+                const uint32_t lo = (uint32_t)__half_as_ushort(__float2half_rn(acc[mt][0][i])) |
+                                    ((uint32_t)__half_as_ushort(__float2half_rn(acc[mt][1][i])) << 16);
+                const uint32_t hi = (uint32_t)__half_as_ushort(__float2half_rn(acc[mt][2][i])) |
+                                    ((uint32_t)__half_as_ushort(__float2half_rn(acc[mt][3][i])) << 16);
+                *reinterpret_cast<uint2*>(C + (long)m * N + n) = make_uint2(lo, hi);
+            }
+        }
+        return;
+    }
+    if (wave == 0) {
+        float* mine = partial + (long)split * M * N;
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) {
+            const int m = 16 * mt + r;
+            if (m >= M) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int n = n0 + 16 * kb + 4 * i;
+                if (n >= N) continue;
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    __hip_atomic_store(mine + (long)m * N + n + j, acc[mt][j][i], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned t = __hip_atomic_fetch_add(&counters[ntile], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last_flag = (t == (unsigned)splits - 1);
+    }
+    __syncthreads();
+    if (!last_flag) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // the last block: 256 threads sum the 64-column x min(M, 16 MT)-row tile
+    const int rows = M < 16 * MT ? M : 16 * MT;
+    for (int e = threadIdx.x; e < rows * kStreamCols; e += 256) {
+        const int m = e / kStreamCols, n = n0 + e % kStreamCols;
+        if (n >= N) continue;
+        float v = 0.0f;
+        for (int sp = 0; sp < splits; sp++)
+            v += __hip_atomic_load(partial + ((long)sp * M + m) * N + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        C[(long)m * N + n] = __float2half_rn(v);
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(&counters[ntile], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void w4a16_tiled_kernel(const __half* __restrict__ A, BW4Params p,
+                                                          __half* __restrict__ C, int M, int N, int K, int tiles_m,
+                                                          int tiles_n) {
+    __shared__ __align__(16) __half smem[2 * (BM + BN) * BK];
+    // tiles in column-major order of (tm, tn) after the XCD remap: consecutive
+    // blocks share the same weight columns
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int tm = t % tiles_m, tn = t / tiles_m;
+    TileOut o;
+    o.C = C;
+    o.ldc = N;
+    o.alpha = 1.0f;
+    o.beta = 0.0f;
+    o.partial = nullptr;
+    o.counter = nullptr;
+    const int slabs = (K + BK - 1) / BK;
+    gemm_tile<BM, BN, BK>(smem, A, (long)K, p, M, N, K, tm * BM, tn * BN, 0, slabs, 0, 1, o);
+}
+
+typedef void (*tiled_fn)(const __half*, BW4Params, __half*, int, int, int, int, int);
+struct TiledEntry {
+    int bm, bn;
+    tiled_fn fn;
+};
+const TiledEntry kTiled[] = {
+    {64, 64, w4a16_tiled_kernel<64, 64, 64>},
+    {64, 128, w4a16_tiled_kernel<64, 128, 64>},
+    {128, 128, w4a16_tiled_kernel<128, 128, 64>},
+    {128, 256, w4a16_tiled_kernel<128, 256, 64>},
+    {256, 128, w4a16_tiled_kernel<256, 128, 64>},
+    {256, 256, w4a16_tiled_kernel<256, 256, 64>},
+};
+constexpr int kNumTiled = sizeof(kTiled) / sizeof(kTiled[0]);
+
+void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_w4a16_config* out) {
+    lthip_w4a16_config c = cfg ? *cfg : lthip_w4a16_config{0, 0, 0, 0};
+    if (c.kind == 0) {
+        if (M <= 64) {
+            c.kind = 1;
+            const long ntiles = (long)((N + kStreamCols - 1) / kStreamCols);
+            const long ksteps = (long)((K / 8 + 3) / 4);
+            long split = (4L * cu_count() + ntiles - 1) / ntiles;
+            if (split > ksteps / 4) split = ksteps / 4;
+            if (split < 1) split = 1;
+            c.split_k = (int)split;
+            c.tile_n = kStreamCols;
+            c.tile_m = 16;
+        } else {
+            c.kind = 2;
+            c.tile_m = M >= 1024 ? 256 : 128;
+            c.tile_n = (M >= 1024 && N >= 4096) ? 256 : 128;
+            c.split_k = 1;
+        }
+    }
+    if (c.split_k < 1) c.split_k = 1;
+    *out = c;
+}
+
+}  // namespace
+}  // namespace lt
+
+extern "C" {
+
+size_t lthip_w4a16_workspace_size(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg) {
+    lthip_w4a16_config c;
+    lt::resolve(M, N, K, cfg, &c);
+    if (c.kind != 1 || c.split_k <= 1) return 0;
+    return lt::kCounterBytes + sizeof(float) * (size_t)c.split_k * M * N;
+}
+
+int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t* scales, const uint32_t* qzeros,
+                     uint16_t* C, size_t M, size_t N, size_t K, int group, const lthip_w4a16_config* cfg, void* ws,
+                     size_t ws_bytes, void* stream) {
+    using namespace lt;
+    if (M > (1u << 30) || N > (1u << 30) || K > (1u << 30)) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: size too large");
+    if (group <= 0 || group % 8 != 0 || K % (size_t)group != 0)
+        return fail(LTHIP_ERR_INVALID_ARG, "w4a16: K must be a multiple of group, group a multiple of 8");
+    if (N % 8 != 0) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: N must be a multiple of 8");
+    if (M == 0 || N == 0) return LTHIP_OK;
+    if (!C || (K > 0 && (!A || !W_packed || !scales))) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: null pointer");
+    if (reinterpret_cast<uintptr_t>(A) % 16 || reinterpret_cast<uintptr_t>(W_packed) % 16 ||
+        reinterpret_cast<uintptr_t>(scales) % 8 || reinterpret_cast<uintptr_t>(C) % 8)
+        return fail(LTHIP_ERR_INVALID_ARG, "w4a16: misaligned pointer");
+    lthip_w4a16_config c;
+    resolve(M, N, K, cfg, &c);
+    BW4Params p{W_packed, reinterpret_cast<const __half*>(scales), qzeros, group};
+    const int m = (int)M, n = (int)N, k = (int)K;
+    hipStream_t s = as_stream(stream);
+    if (c.kind == 1) {
+        if (M > 64) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: kind 1 supports M <= 64");
+        const int ntiles = (n + kStreamCols - 1) / kStreamCols;
+        const int ksteps = (k / 8 + 3) / 4;
+        int splits = c.split_k;
+        if (splits > ksteps) splits = ksteps > 0 ? ksteps : 1;
+        if ((size_t)ntiles > kMaxCounters) splits = 1;
+        float* partial = nullptr;
+        unsigned* counters = nullptr;
+        if (splits > 1) {
+            lthip_w4a16_config c2 = c;
+            c2.split_k = splits;
+            const size_t need = lthip_w4a16_workspace_size(M, N, K, &c2);
+            if (!ws || ws_bytes < need) return fail(LTHIP_ERR_WORKSPACE, "w4a16: need %zu workspace bytes", need);
+            if (reinterpret_cast<uintptr_t>(ws) % 16) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: workspace alignment");
+            counters = reinterpret_cast<unsigned*>(ws);
+            partial = reinterpret_cast<float*>(reinterpret_cast<char*>(ws) + kCounterBytes);
+        }
+        dim3 grid((unsigned)ntiles, (unsigned)splits);
+        const __half* a = reinterpret_cast<const __half*>(A);
+        __half* cc = reinterpret_cast<__half*>(C);
+        const int mt = (m + 15) / 16;
+        if (mt <= 1)
+            hipLaunchKernelGGL(w4a16_stream_kernel<1>, grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial, counters);
+        else if (mt <= 2)
+            hipLaunchKernelGGL(w4a16_stream_kernel<2>, grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial, counters);
+        else
+            hipLaunchKernelGGL(w4a16_stream_kernel<4>, grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial, counters);
+        LT_TRY_LAUNCH("w4a16 stream launch");
+        return LTHIP_OK;
+    }
+    if (c.kind != 2) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: unknown kind %d", c.kind);
+    int idx = -1;
+    for (int i = 0; i < kNumTiled; i++)
+        if (kTiled[i].bm == c.tile_m && kTiled[i].bn == c.tile_n) idx = i;
+    if (idx < 0) return fail(LTHIP_ERR_UNSUPPORTED, "w4a16: tile %dx%d is not built", c.tile_m, c.tile_n);
+    const int tiles_m = (m + c.tile_m - 1) / c.tile_m, tiles_n = (n + c.tile_n - 1) / c.tile_n;
+    hipLaunchKernelGGL(kTiled[idx].fn, dim3((unsigned)(tiles_m * tiles_n)), dim3(256), 0, s,
+                       reinterpret_cast<const __half*>(A), p, reinterpret_cast<__half*>(C), m, n, k, tiles_m, tiles_n);
+    LT_TRY_LAUNCH("w4a16 tiled launch");
+    return LTHIP_OK;
+}
+
+int lthip_w4_dequant(const uint32_t* W_packed, const uint16_t* scales, const uint32_t* qzeros, uint16_t* W, size_t K,
+                     size_t N, int group, void* stream) {
+    using namespace lt;
+    if (group <= 0 || group % 8 != 0 || K % (size_t)group != 0 || N % 8 != 0)
+        return fail(LTHIP_ERR_INVALID_ARG, "w4_dequant: bad K/N/group");
+    if (K == 0 || N == 0) return LTHIP_OK;
+    if (!W_packed || !scales || !W) return fail(LTHIP_ERR_INVALID_ARG, "w4_dequant: null pointer");
+    if ((K / 8) * N > (size_t)1 << 40) return fail(LTHIP_ERR_INVALID_ARG, "w4_dequant: too large");
+    const long total = (long)(K / 8) * (long)N;
+    hipLaunchKernelGGL(dequant_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, as_stream(stream), W_packed,
+                       reinterpret_cast<const __half*>(scales), qzeros, reinterpret_cast<__half*>(W), (int)K, (int)N,
+                       group);
+    LT_TRY_LAUNCH("w4_dequant launch");
+    return LTHIP_OK;
+}
+
+}  // extern "C"

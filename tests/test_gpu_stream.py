@@ -1,0 +1,81 @@
+"""GPU parity: input generation, reduction and sdot against the oracle.
+
+Bars: generators bit-exact; reduction/sdot within rel 1e-5 of the oracle's
+FP64 sum (the configuration's tolerance), exact on integer inputs, bitwise
+deterministic across runs.
+"""
+import numpy as np
+import pytest
+import torch
+
+from libthom_amd import inputs, routines as R
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dtype,npt", [(torch.float32, np.float32), (torch.float16, np.float16),
+                                       (torch.float64, np.float64)])
+def test_fill_uniform_bit_exact(dev, orc, dtype, npt):
+    n = 1_000_003
+    t = torch.empty(n, dtype=dtype, device=dev)
+    R.fill_uniform(t, -1.0, 1.0, 1234, 5)
+    got = t.cpu().numpy()
+    want = orc.fill_uniform(n, -1.0, 1.0, 1234, 5, npt)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
+def test_copy(dev):
+    src = torch.arange(1_000_001, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    R.copy_f32(dst, src)
+    assert torch.equal(dst, src)
+
+
+SIZES = [0, 1, 3, 4, 5, 255, 1000, 65537, 1_000_000, 10_000_019]
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_reduction_vs_oracle(dev, orc, n):
+    x = inputs.uniform_f32(n, 0.0, 1.0, 1234, inputs.S_REDUCE)
+    got = float(R.reduce_sum(torch.from_numpy(x).to(dev)).item())
+    ref = orc.sum_f64(x)
+    if n == 0:
+        assert got == 0.0
+    else:
+        assert abs(got - ref) <= 1e-5 * abs(ref), (got, ref)
+
+
+def test_reduction_signed_and_unaligned(dev, orc):
+    x = inputs.uniform_f32(3_000_001, -3.0, 3.0, 1234, 21)
+    t = torch.from_numpy(x).to(dev)
+    for off in (1, 2, 3):   # misaligned start -> scalar path
+        got = float(R.reduce_sum(t[off:]).item())
+        ref = orc.sum_f64(x[off:])
+        assert abs(got - ref) <= 1e-5 * np.abs(x[off:]).sum()
+
+
+def test_reduction_exact_integers_and_deterministic(dev):
+    n = 1 << 24
+    t = torch.ones(n, dtype=torch.float32, device=dev)
+    assert R.reduce_sum(t).item() == float(n)
+    x = torch.rand(5_000_000, device=dev)
+    a = R.reduce_sum(x).clone()
+    b = R.reduce_sum(x).clone()
+    assert torch.equal(a, b)     # fixed order, and the ticket counter was reset
+
+
+@pytest.mark.parametrize("n", [1, 7, 4096, 1_000_000, 9_999_999])
+def test_sdot_vs_oracle(dev, orc, n):
+    x = inputs.uniform_f32(n, -1.0, 1.0, 1234, inputs.S_SDOT_X)
+    y = inputs.uniform_f32(n, -1.0, 1.0, 1234, inputs.S_SDOT_Y)
+    got = float(R.sdot(torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev)).item())
+    ref = orc.dot_f64(x, y)
+    # signed inputs cancel: scale the tolerance by sum |x y|
+    assert abs(got - ref) <= 1e-5 * float(np.abs(x.astype(np.float64) * y).sum()), (got, ref)
+
+
+def test_sdot_basis_vector(dev):
+    x = torch.rand(100_003, device=dev)
+    e = torch.zeros_like(x)
+    e[77777] = 1.0
+    assert R.sdot(x, e).item() == x[77777].item()

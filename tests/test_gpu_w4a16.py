@@ -1,0 +1,123 @@
+"""GPU parity: int4 dequantisation (bit-exact) and w4a16 against the oracle.
+
+The dequantisation is checked bit for bit over every (q, z, finite scale bit
+pattern).  The product uses the reference's criterion (w4a16.cpp:194-197):
+|r - ref| <= 1e-2 |ref| + 1e-3 RMS(C_ref), against the oracle's FP32
+k-ascending sum over the oracle-dequantised weights.
+"""
+import numpy as np
+import pytest
+import torch
+
+from libthom_amd import W4A16Config, inputs, routines as R
+
+pytestmark = pytest.mark.gpu
+
+
+def t(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def i32(a, dev):
+    return t(np.ascontiguousarray(a).view(np.int32), dev)
+
+
+def test_dequant_exhaustive_bit_exact(dev, orc):
+    group = 16
+    s_bits = np.arange(65536, dtype=np.uint16)
+    s_bits = s_bits[(s_bits & 0x7C00) != 0x7C00]
+    N = s_bits.size * 16
+    N -= N % 8
+    z = np.tile(np.arange(16, dtype=np.uint8), s_bits.size)[None, :N]
+    scales = np.repeat(s_bits, 16)[None, :N].view(np.float16)
+    q = np.repeat(np.arange(16, dtype=np.uint8)[:, None], N, axis=1)
+    packed = orc.w4_pack(q)
+    qz = orc.w4_pack_zeros(z)
+    want = orc.w4_dequant(packed, scales, qz, group)
+    got = R.w4_dequant(i32(packed, dev), t(scales, dev), i32(qz, dev), group).cpu().numpy()
+    assert np.array_equal(got.view(np.uint16), want.view(np.uint16))
+
+
+def test_dequant_golden_and_symmetric(dev, orc):
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "w4_tiny.json")))
+    packed = np.array(g["packed"], dtype=np.uint32)
+    scales = np.array(g["scales_bits"], dtype=np.uint16).view(np.float16)
+    qz = np.array(g["qzeros"], dtype=np.uint32)
+    got = R.w4_dequant(i32(packed, dev), t(scales, dev), i32(qz, dev), g["group"]).cpu().numpy()
+    assert got.view(np.uint16).tolist() == g["dequant_bits"]
+    q = np.random.default_rng(1).integers(0, 16, (256, 64)).astype(np.uint8)
+    s = np.random.default_rng(2).uniform(0.005, 0.02, (2, 64)).astype(np.float16)
+    p = orc.w4_pack(q)
+    got = R.w4_dequant(i32(p, dev), t(s, dev), None, 128).cpu().numpy()
+    assert np.array_equal(got, orc.w4_dequant(p, s, None, 128))
+
+
+def w4_close(got, ref):
+    got = got.astype(np.float32)
+    ref = ref.astype(np.float32)
+    rms = float(np.sqrt(np.mean(ref.astype(np.float64) ** 2)))
+    bad = np.abs(got - ref) > 1e-2 * np.abs(ref) + 1e-3 * rms
+    assert not bad.any(), f"{bad.sum()} of {bad.size} outside 1% + 1e-3 RMS; max abs err {np.abs(got - ref).max()}"
+
+
+_PROBLEMS = {}
+
+
+def problem(M, N=11008, K=4096, group=128):
+    key = (M, N, K, group)
+    if key not in _PROBLEMS:
+        A, q, s, z = inputs.w4a16_problem(M, N, K, group)
+        _PROBLEMS[key] = (A, inputs.pack_w4(q), s, inputs.pack_zeros(z))
+    return _PROBLEMS[key]
+
+
+@pytest.mark.parametrize("M", [1, 16, 128, 2048])
+def test_w4a16_config_shapes(dev, orc, M):
+    N, K, group = 11008, 4096, 128
+    A, packed, s, qz = problem(M)
+    C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), group).cpu().numpy()
+    Wdq = orc.w4_dequant(packed, s, qz, group)
+    rows = M if M <= 128 else 64
+    row0 = 0 if M <= 128 else M - 64
+    ref = orc.w4a16_rows(A, Wdq, row0, rows)
+    w4_close(C[row0:row0 + rows], ref)
+
+
+@pytest.mark.parametrize("cfg", [W4A16Config(1, 64, 16, 1), W4A16Config(1, 64, 16, 4), W4A16Config(1, 64, 16, 32),
+                                 W4A16Config(2, 64, 64, 1), W4A16Config(2, 128, 128, 1),
+                                 W4A16Config(2, 256, 256, 1), W4A16Config(2, 128, 256, 1)])
+@pytest.mark.parametrize("M", [1, 5, 16, 33, 64])
+def test_w4a16_kinds(dev, orc, cfg, M):
+    N, K, group = 520, 1024, 128
+    A, q, s, z = inputs.w4a16_problem(M, N, K, group, seed=77)
+    packed, qz = inputs.pack_w4(q), inputs.pack_zeros(z)
+    C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), group, config=cfg).cpu().numpy()
+    ref = orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, group))
+    w4_close(C, ref)
+
+
+def test_w4a16_exact_small_integers(dev, orc):
+    # scale 1, A small integers: every product and sum is exact in FP32
+    M, N, K = 20, 64, 256
+    rng = np.random.default_rng(5)
+    q = rng.integers(0, 16, (K, N)).astype(np.uint8)
+    z = rng.integers(0, 16, (K // 128, N)).astype(np.uint8)
+    s = np.ones((K // 128, N), np.float16)
+    A = rng.integers(-2, 3, (M, K)).astype(np.float16)
+    packed, qz = orc.w4_pack(q), orc.w4_pack_zeros(z)
+    ref = orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, 128))
+    for cfg in (W4A16Config(1, 64, 16, 1), W4A16Config(1, 64, 16, 4), W4A16Config(2, 64, 64, 1)):
+        C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), 128, config=cfg).cpu().numpy()
+        assert np.array_equal(C, ref), cfg
+
+
+def test_w4a16_symmetric_zero_point(dev, orc):
+    M, N, K = 16, 256, 512
+    A, q, s, _ = inputs.w4a16_problem(M, N, K, 128, seed=3)
+    packed = inputs.pack_w4(q)
+    ref = orc.w4a16_rows(A, orc.w4_dequant(packed, s, None, 128))
+    for cfg in (None, W4A16Config(2, 64, 64, 1)):
+        C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), None, 128, config=cfg).cpu().numpy()
+        w4_close(C, ref)
