@@ -92,6 +92,26 @@ int lthip_sgemv_f32(int row_major, int m, int n, const float *alpha, const float
 int lthip_dgemm_f64(const double *A, const double *B, double *C, size_t M, size_t N, size_t K,
                     double alpha, double beta, void *stream);
 
+/* ---- split-precision (Ozaki-style) dgemm on FP8 MFMA.  No counterpart in the
+ *      reference: it is the emulated alternative to DgemmImpl (dgemm.cpp:36),
+ *      same operands and C = alpha*A*B + beta*C.  Rows of A and columns of B are
+ *      scaled by powers of two and cut into `slices` signed base-32 digits
+ *      (E4M3-exact, |d| <= 16); slice products run on
+ *      v_mfma_scale_f32_16x16x128_f8f6f4 with exact FP32 level sums and are
+ *      recombined in FP64.  2 <= slices <= 10, K % 16 == 0 and
+ *      slices * 256 * K < 2^24 (lthip_ozaki_max_slices(K) gives the limit). */
+int lthip_ozaki_max_slices(size_t K);
+size_t lthip_dgemm_ozaki_workspace_size(size_t M, size_t N, size_t K, int slices);
+int lthip_dgemm_ozaki(const double *A, const double *B, double *C, size_t M, size_t N, size_t K,
+                      double alpha, double beta, int slices, void *ws, size_t ws_bytes, void *stream);
+/* The split itself (format check): digits[p][r][c] as E4M3 bytes, exps[r]. */
+int lthip_ozaki_split_rows(const double *X, size_t rows, size_t cols, size_t ld, int slices,
+                           uint8_t *digits, int32_t *exps, void *stream);
+/* Compensated (Dot2) FP64 GEMM, C_hi + C_lo ~ A*B to about twice FP64
+ * precision: the "FP64 oracle" on the GPU for measuring dgemm errors. */
+int lthip_dgemm_compensated(const double *A, const double *B, double *C_hi, double *C_lo, size_t M,
+                            size_t N, size_t K, void *stream);
+
 /* ---- hgemm: replaces HgemmFunction (hgemm.h:6-7; hgemm_better_reuse.cu:357).
  *      Row-major A (M x K), B (K x N), C (M x N) fp16 (bit patterns);
  *      C = fp16(alpha*A*B + beta*C) with FP32 accumulation on
@@ -115,6 +135,36 @@ size_t lthip_hgemm_workspace_size(size_t M, size_t N, size_t K, const lthip_hgem
 int lthip_hgemm_f16(const uint16_t *A, const uint16_t *B, uint16_t *C, size_t M, size_t N, size_t K,
                     uint16_t alpha, uint16_t beta, const lthip_hgemm_config *cfg, void *ws,
                     size_t ws_bytes, void *stream);
+
+/* ---- grouped / batched hgemm (no counterpart in the reference, which runs
+ *      one GEMM per launch): `count` independent row-major problems of any
+ *      shapes (K, N, lda, ldb multiples of 8) in ONE persistent launch of one
+ *      tile shape `cfg`; each problem picks its own split_k (0 or 1 = none).
+ *      plan_create uploads the problem table into the workspace (blocking,
+ *      once); run launches it; capture records that launch into a hipGraph on a
+ *      non-null stream and graph_launch replays it.  The launch leaves its work
+ *      queue and split-K counters at zero, so plans and graphs replay as is.
+ *      Problems sharing an output must not be in one plan. */
+typedef struct lthip_hgemm_problem {
+    const uint16_t *A, *B;
+    uint16_t *C;
+    int M, N, K;
+    int lda, ldb, ldc;   /* 0 = packed (K, N, N) */
+    int split_k;         /* 0/1 = no split */
+} lthip_hgemm_problem;
+typedef struct lthip_grouped_plan lthip_grouped_plan;
+
+size_t lthip_hgemm_grouped_workspace_size(const lthip_hgemm_problem *probs, int count,
+                                          const lthip_hgemm_config *cfg);
+int lthip_hgemm_grouped_plan_create(const lthip_hgemm_problem *probs, int count,
+                                    const lthip_hgemm_config *cfg, uint16_t alpha, uint16_t beta,
+                                    void *ws, size_t ws_bytes, void *stream,
+                                    lthip_grouped_plan **plan);
+int lthip_hgemm_grouped_run(lthip_grouped_plan *plan, void *stream);
+int lthip_hgemm_grouped_capture(lthip_grouped_plan *plan, void *stream);
+int lthip_hgemm_grouped_graph_launch(lthip_grouped_plan *plan, void *stream);
+int lthip_hgemm_grouped_plan_info(const lthip_grouped_plan *plan, int *work_items, int *grid);
+int lthip_hgemm_grouped_plan_destroy(lthip_grouped_plan *plan);
 
 /* ---- w4a16: replaces W4A16Function float(*)(const W4A16Problem&) (w4a16.h:23-34;
  *      w4a16_naive.cu:90) with per-(group, column) zero-points added.

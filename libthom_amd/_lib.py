@@ -35,6 +35,11 @@ class HgemmConfig(ctypes.Structure):
         return f"HgemmConfig(bm={self.bm}, bn={self.bn}, bk={self.bk}, split_k={self.split_k}, stages={self.stages})"
 
 
+class HgemmProblem(ctypes.Structure):
+    _fields_ = [("A", c_vp), ("B", c_vp), ("C", c_vp), ("M", c_int), ("N", c_int), ("K", c_int),
+                ("lda", c_int), ("ldb", c_int), ("ldc", c_int), ("split_k", c_int)]
+
+
 class W4A16Config(ctypes.Structure):
     _fields_ = [("kind", c_int), ("tile_n", c_int), ("tile_m", c_int), ("split_k", c_int)]
 
@@ -63,12 +68,26 @@ _SIGS = {
     "lthip_sgemv_f32": (c_int, [c_int, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_int,
                                 c_vp, c_size, c_vp]),
     "lthip_dgemm_f64": (c_int, [c_vp, c_vp, c_vp, c_size, c_size, c_size, c_dbl, c_dbl, c_vp]),
+    "lthip_ozaki_max_slices": (c_int, [c_size]),
+    "lthip_dgemm_ozaki_workspace_size": (c_size, [c_size, c_size, c_size, c_int]),
+    "lthip_dgemm_ozaki": (c_int, [c_vp, c_vp, c_vp, c_size, c_size, c_size, c_dbl, c_dbl, c_int, c_vp, c_size,
+                                  c_vp]),
+    "lthip_ozaki_split_rows": (c_int, [c_vp, c_size, c_size, c_size, c_int, c_vp, c_vp, c_vp]),
+    "lthip_dgemm_compensated": (c_int, [c_vp, c_vp, c_vp, c_vp, c_size, c_size, c_size, c_vp]),
     "lthip_hgemm_config_count": (c_int, []),
     "lthip_hgemm_config_get": (c_int, [c_int, ctypes.POINTER(HgemmConfig)]),
     "lthip_hgemm_default_config": (c_int, [c_size, c_size, c_size, ctypes.POINTER(HgemmConfig)]),
     "lthip_hgemm_workspace_size": (c_size, [c_size, c_size, c_size, ctypes.POINTER(HgemmConfig)]),
     "lthip_hgemm_f16": (c_int, [c_vp, c_vp, c_vp, c_size, c_size, c_size, c_u16, c_u16,
                                 ctypes.POINTER(HgemmConfig), c_vp, c_size, c_vp]),
+    "lthip_hgemm_grouped_workspace_size": (c_size, [c_vp, c_int, ctypes.POINTER(HgemmConfig)]),
+    "lthip_hgemm_grouped_plan_create": (c_int, [c_vp, c_int, ctypes.POINTER(HgemmConfig), c_u16, c_u16, c_vp,
+                                                c_size, c_vp, ctypes.POINTER(c_vp)]),
+    "lthip_hgemm_grouped_run": (c_int, [c_vp, c_vp]),
+    "lthip_hgemm_grouped_capture": (c_int, [c_vp, c_vp]),
+    "lthip_hgemm_grouped_graph_launch": (c_int, [c_vp, c_vp]),
+    "lthip_hgemm_grouped_plan_info": (c_int, [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+    "lthip_hgemm_grouped_plan_destroy": (c_int, [c_vp]),
     "lthip_w4a16_workspace_size": (c_size, [c_size, c_size, c_size, ctypes.POINTER(W4A16Config)]),
     "lthip_w4a16_gemm": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_size, c_size, c_int,
                                  ctypes.POINTER(W4A16Config), c_vp, c_size, c_vp]),

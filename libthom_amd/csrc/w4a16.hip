@@ -8,7 +8,7 @@
 // Because 8 consecutive k of one column live in one uint32, a dequantised
 // uint32 is exactly one MFMA operand fragment (8 halves along K).
 //
-// kind 1, small M (decode, M <= 64): MFMA 16x16x32 on C^T = W^T A^T.  A wave
+// kind 1, small M (decode, M <= 128): MFMA 16x16x32 on C^T = W^T A^T.  A wave
 // owns 64 columns; lane l loads the uint4 W[kp][n0 + 4(l%16) .. +3] (kp = k-block
 // l/16), so MFMA tile j covers columns n0 + 4r + j.  The activations are the
 // B operand (16 rows of A per MFMA).  The four waves of a block split the
@@ -45,21 +45,25 @@ __global__ __launch_bounds__(256) void dequant_kernel(const uint32_t* __restrict
 
 constexpr int kStreamCols = 64;
 
-template <int MT>
+// U k-steps per wave are loaded before any is used: every load is
+// unconditional (out-of-range indices are clamped to a valid address and the
+// activation operand is zeroed instead), so all of them are in flight at once.
+template <int MT, int U>
 __global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restrict__ A, BW4Params p,
                                                            __half* __restrict__ C, int M, int N, int K, int splits,
                                                            float* __restrict__ partial, unsigned* __restrict__ counters) {
-    __shared__ float red[3][MT * 16][64];
+    __shared__ float red[3][16][64];
     __shared__ int last_flag;
     const int ntile = blockIdx.x, split = blockIdx.y;
     const int n0 = ntile * kStreamCols;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, kb = lane >> 4;
     const int ncol = n0 + 4 * r;
-    const bool colv = ncol < N;
+    const int ncolc = ncol < N ? ncol : N - 4;          // clamped: loads stay in bounds, results unused
     const int KP = K / 8;
     const int ksteps = (KP + 3) / 4;   // 32 k per MFMA step
     const int ks0 = (int)((long)ksteps * split / splits), ks1 = (int)((long)ksteps * (split + 1) / splits);
+    const long zrow = N / 8;
 
     v4f acc[MT][4];
 #pragma unroll
@@ -67,63 +71,73 @@ __global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restr
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[mt][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-    for (int ks = ks0 + wave; ks < ks1; ks += 4) {
-        const int kp = ks * 4 + kb;
-        const bool kv = kp < KP;
-        uint4 w = make_uint4(0, 0, 0, 0);
-        uint2 s = make_uint2(0, 0);
-        uint32_t z = 0;
-        if (kv && colv) {
-            const int g = kp * 8 / p.group;
-            w = ld_nt(reinterpret_cast<const uint4*>(p.W + (long)kp * N + ncol));
-            s = *reinterpret_cast<const uint2*>(p.scales + (long)g * N + ncol);
-            z = p.qzeros ? (p.qzeros[(long)g * (N / 8) + ncol / 8] >> (4 * (ncol & 7))) & 0xFFFFu : 0x8888u;
+    for (int ks = ks0 + wave; ks < ks1; ks += 4 * U) {
+        uint4 w[U];
+        uint2 s[U];
+        uint32_t z[U];
+        uint4 a[U][MT];
+        bool kv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int kp = (ks + 4 * u) * 4 + kb;
+            kv[u] = (ks + 4 * u < ks1) && (kp < KP);
+            const int kpc = kv[u] ? kp : 0;
+            const int g = kpc * 8 / p.group;
+            w[u] = ld_nt(reinterpret_cast<const uint4*>(p.W + (long)kpc * N + ncolc));
+            s[u] = *reinterpret_cast<const uint2*>(p.scales + (long)g * N + ncolc);
+            z[u] = p.qzeros ? p.qzeros[(long)g * zrow + ncolc / 8] : 0x88888888u;
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) {
+                const int m = 16 * mt + r;
+                const int mc = m < M ? m : M - 1;
+                a[u][mt] = *reinterpret_cast<const uint4*>(A + (long)mc * K + 8 * kpc);
+            }
         }
-        const uint32_t wv[4] = {w.x, w.y, w.z, w.w};
-        const uint32_t sv[4] = {s.x & 0xFFFFu, s.x >> 16, s.y & 0xFFFFu, s.y >> 16};
-        v8h wf[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint32_t zj = 0x6400u + ((z >> (4 * j)) & 0xFu);
-            const uint4 d = dequant8(wv[j], zj | (zj << 16), sv[j] | (sv[j] << 16));
-            __builtin_memcpy(&wf[j], &d, 16);
-        }
+        for (int u = 0; u < U; u++) {
+            const uint32_t zs = (z[u] >> (4 * (ncolc & 7))) & 0xFFFFu;
+            const uint32_t wv[4] = {w[u].x, w[u].y, w[u].z, w[u].w};
+            const uint32_t sv[4] = {s[u].x & 0xFFFFu, s[u].x >> 16, s[u].y & 0xFFFFu, s[u].y >> 16};
+            v8h wf[4];
 #pragma unroll
-        for (int mt = 0; mt < MT; mt++) {
-            const int m = 16 * mt + r;
-            uint4 a = make_uint4(0, 0, 0, 0);
-            if (kv && m < M) a = *reinterpret_cast<const uint4*>(A + (long)m * K + 8 * kp);
-            a = perm_k8(a);
-            v8h af;
-            __builtin_memcpy(&af, &a, 16);
+            for (int j = 0; j < 4; j++) {
+                const uint32_t zj = 0x6400u + ((zs >> (4 * j)) & 0xFu);
+                const uint4 d = dequant8(wv[j], zj | (zj << 16), sv[j] | (sv[j] << 16));
+                __builtin_memcpy(&wf[j], &d, 16);
+            }
 #pragma unroll
-            for (int j = 0; j < 4; j++) acc[mt][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wf[j], af, acc[mt][j], 0, 0, 0);
+            for (int mt = 0; mt < MT; mt++) {
+                const bool use = kv[u] && (16 * mt + r < M);
+                uint4 av = perm_k8(a[u][mt]);
+                if (!use) av = make_uint4(0, 0, 0, 0);
+                v8h af;
+                __builtin_memcpy(&af, &av, 16);
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[mt][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wf[j], af, acc[mt][j], 0, 0, 0);
+            }
         }
     }
 
-    // cross-wave sum in wave order (deterministic): waves 1..3 park, wave 0 adds
-    if (wave > 0) {
+    // cross-wave sum in wave order (deterministic), one 16-row m-tile at a time
 #pragma unroll
-        for (int mt = 0; mt < MT; mt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) red[wave - 1][mt * 16 + j * 4 + i][lane] = acc[mt][j][i];
-    }
-    __syncthreads();
-    if (wave == 0) {
-#pragma unroll
-        for (int mt = 0; mt < MT; mt++)
+    for (int mt = 0; mt < MT; mt++) {
+        if (wave > 0) {
 #pragma unroll
             for (int j = 0; j < 4; j++)
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    float v = acc[mt][j][i];
-                    v += red[0][mt * 16 + j * 4 + i][lane];
-                    v += red[1][mt * 16 + j * 4 + i][lane];
-                    v += red[2][mt * 16 + j * 4 + i][lane];
-                    acc[mt][j][i] = v;
-                }
+                for (int i = 0; i < 4; i++) red[wave - 1][j * 4 + i][lane] = acc[mt][j][i];
+        }
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    acc[mt][j][i] = ((acc[mt][j][i] + red[0][j * 4 + i][lane]) + red[1][j * 4 + i][lane]) +
+                                    red[2][j * 4 + i][lane];
+        }
+        __syncthreads();
     }
     // acc[mt][j][i] of lane l is C[m = 16 mt + l%16][n = n0 + 16 (l/16) + 4 i + j]
     if (splits <= 1) {
@@ -155,10 +169,8 @@ __global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restr
             for (int i = 0; i < 4; i++) {
                 const int n = n0 + 16 * kb + 4 * i;
                 if (n >= N) continue;
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    __hip_atomic_store(mine + (long)m * N + n + j, acc[mt][j][i], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+                *reinterpret_cast<v4f*>(mine + (long)m * N + n) =
+                    v4f{acc[mt][0][i], acc[mt][1][i], acc[mt][2][i], acc[mt][3][i]};
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -171,15 +183,20 @@ __global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restr
     __syncthreads();
     if (!last_flag) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    // the last block: 256 threads sum the 64-column x min(M, 16 MT)-row tile
+    // the last block: 256 threads sum the 64-column x min(M, 16 MT)-row tile, 4 columns each
     const int rows = M < 16 * MT ? M : 16 * MT;
-    for (int e = threadIdx.x; e < rows * kStreamCols; e += 256) {
-        const int m = e / kStreamCols, n = n0 + e % kStreamCols;
+    for (int e = threadIdx.x; e < rows * (kStreamCols / 4); e += 256) {
+        const int m = e / (kStreamCols / 4), n = n0 + 4 * (e % (kStreamCols / 4));
         if (n >= N) continue;
-        float v = 0.0f;
-        for (int sp = 0; sp < splits; sp++)
-            v += __hip_atomic_load(partial + ((long)sp * M + m) * N + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        C[(long)m * N + n] = __float2half_rn(v);
+        v4f v = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int sp = 0; sp < splits; sp++) {
+            v += *reinterpret_cast<const v4f*>(partial + ((long)sp * M + m) * N + n);
+        }
+        const uint32_t lo = (uint32_t)__half_as_ushort(__float2half_rn(v[0])) |
+                            ((uint32_t)__half_as_ushort(__float2half_rn(v[1])) << 16);
+        const uint32_t hi = (uint32_t)__half_as_ushort(__float2half_rn(v[2])) |
+                            ((uint32_t)__half_as_ushort(__float2half_rn(v[3])) << 16);
+        *reinterpret_cast<uint2*>(C + (long)m * N + n) = make_uint2(lo, hi);
     }
     if (threadIdx.x == 0) __hip_atomic_store(&counters[ntile], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -222,7 +239,7 @@ constexpr int kNumTiled = sizeof(kTiled) / sizeof(kTiled[0]);
 void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_w4a16_config* out) {
     lthip_w4a16_config c = cfg ? *cfg : lthip_w4a16_config{0, 0, 0, 0};
     if (c.kind == 0) {
-        if (M <= 64) {
+        if (M <= 128) {
             c.kind = 1;
             const long ntiles = (long)((N + kStreamCols - 1) / kStreamCols);
             const long ksteps = (long)((K / 8 + 3) / 4);
@@ -274,7 +291,7 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
     const int m = (int)M, n = (int)N, k = (int)K;
     hipStream_t s = as_stream(stream);
     if (c.kind == 1) {
-        if (M > 64) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: kind 1 supports M <= 64");
+        if (M > 128) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: kind 1 supports M <= 128");
         const int ntiles = (n + kStreamCols - 1) / kStreamCols;
         const int ksteps = (k / 8 + 3) / 4;
         int splits = c.split_k;
@@ -296,11 +313,17 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
         __half* cc = reinterpret_cast<__half*>(C);
         const int mt = (m + 15) / 16;
         if (mt <= 1)
-            hipLaunchKernelGGL(w4a16_stream_kernel<1>, grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial, counters);
+            hipLaunchKernelGGL((w4a16_stream_kernel<1, 4>), grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial,
+                               counters);
         else if (mt <= 2)
-            hipLaunchKernelGGL(w4a16_stream_kernel<2>, grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial, counters);
+            hipLaunchKernelGGL((w4a16_stream_kernel<2, 4>), grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial,
+                               counters);
+        else if (mt <= 4)
+            hipLaunchKernelGGL((w4a16_stream_kernel<4, 2>), grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial,
+                               counters);
         else
-            hipLaunchKernelGGL(w4a16_stream_kernel<4>, grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial, counters);
+            hipLaunchKernelGGL((w4a16_stream_kernel<8, 2>), grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial,
+                               counters);
         LT_TRY_LAUNCH("w4a16 stream launch");
         return LTHIP_OK;
     }

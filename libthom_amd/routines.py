@@ -130,6 +130,45 @@ def dgemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, al
     return C
 
 
+def dgemm_ozaki(A: torch.Tensor, B: torch.Tensor, slices: int, C: Optional[torch.Tensor] = None,
+                alpha: float = 1.0, beta: float = 0.0) -> torch.Tensor:
+    """Split-precision dgemm on FP8 MFMA with `slices` digits per operand."""
+    _need_cuda(A, B)
+    M, K = A.shape
+    K2, N = B.shape
+    assert K == K2 and A.dtype == torch.float64 and B.dtype == torch.float64
+    if C is None:
+        C = torch.zeros(M, N, dtype=torch.float64, device=A.device)
+    need = lib().lthip_dgemm_ozaki_workspace_size(M, N, K, slices)
+    ws = workspace(need, A.device, "ozaki")
+    call("lthip_dgemm_ozaki", A.data_ptr(), B.data_ptr(), C.data_ptr(), M, N, K, float(alpha), float(beta), slices,
+         ws.data_ptr(), ws.numel(), _stream(A.device))
+    return C
+
+
+def ozaki_split_rows(X: torch.Tensor, slices: int):
+    """(digits[slices][rows][cols] E4M3 bytes, exps[rows]) of a row-major FP64 matrix."""
+    _need_cuda(X)
+    rows, cols = X.shape
+    d = torch.empty(slices, rows, cols, dtype=torch.uint8, device=X.device)
+    e = torch.empty(rows, dtype=torch.int32, device=X.device)
+    call("lthip_ozaki_split_rows", X.data_ptr(), rows, cols, X.stride(0), slices, d.data_ptr(), e.data_ptr(),
+         _stream(X.device))
+    return d, e
+
+
+def dgemm_compensated(A: torch.Tensor, B: torch.Tensor):
+    """(hi, lo) with hi + lo ~ A@B to about twice FP64 precision (Dot2)."""
+    _need_cuda(A, B)
+    M, K = A.shape
+    N = B.shape[1]
+    hi = torch.empty(M, N, dtype=torch.float64, device=A.device)
+    lo = torch.empty(M, N, dtype=torch.float64, device=A.device)
+    call("lthip_dgemm_compensated", A.data_ptr(), B.data_ptr(), hi.data_ptr(), lo.data_ptr(), M, N, K,
+         _stream(A.device))
+    return hi, lo
+
+
 def _f16_bits(v: float) -> int:
     import numpy as np
     return int(np.array([v], dtype=np.float16).view(np.uint16)[0])
@@ -197,3 +236,66 @@ def w4_dequant(W_packed: torch.Tensor, scales: torch.Tensor, qzeros: Optional[to
     call("lthip_w4_dequant", W_packed.data_ptr(), scales.data_ptr(), _ptr(qzeros), out.data_ptr(), K, N, group,
          _stream(W_packed.device))
     return out
+
+
+class GroupedHgemm:
+    """A plan for many independent hgemms in one persistent launch.
+
+    problems: list of (A, B, C, split_k) fp16 tensors (row-major, C written in
+    place); config: one tile shape for the whole batch.  run() launches on the
+    current stream; capture(stream) records the launch into a hipGraph on a
+    side stream and replay(stream) relaunches it."""
+
+    def __init__(self, problems, config: HgemmConfig, alpha: float = 1.0, beta: float = 0.0):
+        from ._lib import HgemmProblem
+        self._keep = problems
+        n = len(problems)
+        arr = (HgemmProblem * n)()
+        self.device = problems[0][0].device
+        for i, (A, B, C, split) in enumerate(problems):
+            M, K = A.shape
+            N = B.shape[1]
+            arr[i] = HgemmProblem(A.data_ptr(), B.data_ptr(), C.data_ptr(), M, N, K, A.stride(0), B.stride(0),
+                                  C.stride(0), int(split))
+        self._arr = arr
+        self.config = config
+        need = lib().lthip_hgemm_grouped_workspace_size(ctypes.byref(arr), n, ctypes.byref(config))
+        self.ws = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
+        call("lthip_workspace_clear", self.ws.data_ptr(), self.ws.numel(), _stream(self.device))
+        self._plan = ctypes.c_void_p()
+        call("lthip_hgemm_grouped_plan_create", ctypes.byref(arr), n, ctypes.byref(config), _f16_bits(alpha),
+             _f16_bits(beta), self.ws.data_ptr(), self.ws.numel(), _stream(self.device), ctypes.byref(self._plan))
+
+    def info(self):
+        w, g = ctypes.c_int(), ctypes.c_int()
+        call("lthip_hgemm_grouped_plan_info", self._plan, ctypes.byref(w), ctypes.byref(g))
+        return {"work_items": w.value, "grid": g.value}
+
+    def run(self, stream: Optional[torch.cuda.Stream] = None):
+        s = (stream or torch.cuda.current_stream(self.device)).cuda_stream
+        call("lthip_hgemm_grouped_run", self._plan, s)
+
+    def capture(self, stream: torch.cuda.Stream):
+        call("lthip_hgemm_grouped_capture", self._plan, stream.cuda_stream)
+
+    def replay(self, stream: torch.cuda.Stream):
+        call("lthip_hgemm_grouped_graph_launch", self._plan, stream.cuda_stream)
+
+    def close(self):
+        if self._plan:
+            lib().lthip_hgemm_grouped_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batched_problem_dims(count: int = 256, seed: int = 1234):
+    """The batched configuration: count (M, N, K), each log-uniform in
+    [64, 4096] rounded to a multiple of 16."""
+    from . import inputs
+    d = inputs.log_uniform_dims(3 * count, seed=seed)
+    return [(int(d[3 * i]), int(d[3 * i + 1]), int(d[3 * i + 2])) for i in range(count)]
