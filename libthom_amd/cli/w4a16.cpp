@@ -1,0 +1,348 @@
+// w4a16 routine driver (the reference's routines/w4a16/w4a16.cpp):
+// C = A * dequant(W) with int4 weights.  Inputs follow the configured format
+// (uniform nibbles, FP16 scales in [0.005, 0.02], per-(group, column) int4
+// zero-points); a trailing "sym" token instead quantizes fp16 weights with the
+// reference's symmetric quantizer (quantize.cpp:8-53, zero-point 8).
+// Implementations: "hip" (size-based choice), "stream" (split-K decode kernel),
+// "tiled" (MFMA tiles, dequantized into LDS), "tuned" (tuning database, else
+// "hip").  `tune w4a16 M N K` sweeps every configuration into the database.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+
+#include "host_math.h"
+#include "routines.h"
+#include "tuning.h"
+
+namespace w4a16 {
+
+#define BENCH_RUNS 20
+#define WARMUP_RUNS 5
+#define GROUP_SIZE 128
+// w4a16.cpp:17-27 (M sweep at K = N = 8192) and the configured shapes
+static const std::vector<std::vector<int>> SIZES = {
+    {1, 8192, 8192},  {2, 8192, 8192},  {4, 8192, 8192},   {8, 8192, 8192},    {16, 8192, 8192},
+    {32, 8192, 8192}, {64, 8192, 8192}, {128, 8192, 8192}, {256, 8192, 8192},  {1, 11008, 4096},
+    {16, 11008, 4096}, {128, 11008, 4096}, {2048, 11008, 4096}};
+
+struct Problem {   // the reference's W4A16Problem (w4a16.h:23-32) plus zero-points
+    const uint16_t* A;
+    const uint32_t* W;
+    const uint16_t* scales;
+    const uint32_t* qzeros;
+    uint16_t* C;
+    size_t M, N, K;
+};
+using W4A16Function = float (*)(const Problem&);
+
+static float run_cfg(const Problem& p, const lthip_w4a16_config* cfg) {
+    const size_t need = lthip_w4a16_workspace_size(p.M, p.N, p.K, cfg);
+    void* ws = need ? cli::workspace("w4a16", need) : nullptr;
+    return cli::time_ms([&] {
+        LTHIP_CHECK(lthip_w4a16_gemm(p.A, p.W, p.scales, p.qzeros, p.C, p.M, p.N, p.K, GROUP_SIZE, cfg, ws, need,
+                                     cli::stream()));
+    });
+}
+static float w4a16_hip(const Problem& p) { return run_cfg(p, nullptr); }
+static float w4a16_stream(const Problem& p) {
+    lthip_w4a16_config c{1, 64, 16, 0};
+    const long ntiles = (long)(p.N + 63) / 64;
+    c.split_k = (int)std::max(1L, std::min<long>(8, (4L * 256 + ntiles - 1) / ntiles));
+    if (p.M > 128) return run_cfg(p, nullptr);
+    return run_cfg(p, &c);
+}
+static float w4a16_tiled(const Problem& p) {
+    lthip_w4a16_config c{2, 128, 128, 1};
+    return run_cfg(p, &c);
+}
+static float w4a16_tuned(const Problem& p) {
+    lthip_w4a16_config c;
+    if (!tuning::lookup_w4a16((int)p.M, (int)p.N, (int)p.K, GROUP_SIZE, &c)) return run_cfg(p, nullptr);
+    return run_cfg(p, &c);
+}
+
+static const std::unordered_map<std::string, W4A16Function> w4a16Impls = {
+#if LIB_HAS(W4A16_HIP)
+    {"hip", w4a16_hip}, {"stream", w4a16_stream}, {"tiled", w4a16_tiled}, {"tuned", w4a16_tuned},
+#endif
+};
+
+static cli::DevBuf<uint16_t> gpuA, gpuScales, gpuC;
+static cli::DevBuf<uint32_t> gpuW, gpuZ;
+static std::vector<uint16_t> cpuA, cpuScales, cpuWdq, cpuCref;
+static bool symmetric = false;
+
+// quantize.cpp:8-37: symmetric per-(group, column) quantization of fp16 W
+static void quantize_w4(const std::vector<uint16_t>& W, std::vector<uint8_t>& q, std::vector<uint16_t>& scales, int K,
+                        int N) {
+    const int groups = K / GROUP_SIZE;
+    std::vector<float> mx(N);
+    for (int g = 0; g < groups; g++) {
+        std::fill(mx.begin(), mx.end(), 0.0f);
+        for (int k = g * GROUP_SIZE; k < (g + 1) * GROUP_SIZE; k++)
+            for (int n = 0; n < N; n++) mx[n] = std::max(mx[n], std::fabs(cli::f16_to_f32(W[(size_t)k * N + n])));
+        for (int n = 0; n < N; n++) {
+            const float s = 2.0f * mx[n] / 15.0f;
+            scales[(size_t)g * N + n] = cli::f32_to_f16(s > 0.0f ? s : 1.0f);
+        }
+        for (int k = g * GROUP_SIZE; k < (g + 1) * GROUP_SIZE; k++)
+            for (int n = 0; n < N; n++) {
+                const float s = cli::f16_to_f32(scales[(size_t)g * N + n]);
+                const int v = (int)std::nearbyint(cli::f16_to_f32(W[(size_t)k * N + n]) / s) + 8;
+                q[(size_t)k * N + n] = (uint8_t)std::min(std::max(v, 0), 15);
+            }
+    }
+}
+
+// builds the problem on the host; the CPU check uses the dequantized weights
+static void initProblem(int M, int N, int K) {
+    const int G = K / GROUP_SIZE;
+    cpuA.resize((size_t)M * K);
+    for (size_t i = 0; i < cpuA.size(); i++) cpuA[i] = cli::f32_to_f16((float)cli::rng_normal(cli::kSeed, 9, i));
+    std::vector<uint8_t> q((size_t)K * N), z((size_t)G * N);
+    cpuScales.resize((size_t)G * N);
+    if (symmetric) {
+        std::vector<uint16_t> W((size_t)K * N);
+        for (size_t i = 0; i < W.size(); i++)
+            W[i] = cli::f32_to_f16((float)(2.0 * cli::rng_unit(cli::kSeed, 17, i) - 1.0));
+        quantize_w4(W, q, cpuScales, K, N);
+        std::fill(z.begin(), z.end(), 8);
+    } else {
+        cli::parallel_for(q.size(), [&](size_t lo, size_t hi) {
+            for (size_t i = lo; i < hi; i++) q[i] = (uint8_t)cli::rng_int(cli::kSeed, 10, i, 0, 15);
+        });
+        for (size_t i = 0; i < cpuScales.size(); i++)
+            cpuScales[i] = cli::f32_to_f16((float)(0.005 + 0.015 * cli::rng_unit(cli::kSeed, 11, i)));
+        for (size_t i = 0; i < z.size(); i++) z[i] = (uint8_t)cli::rng_int(cli::kSeed, 12, i, 0, 15);
+    }
+    // pack (quantize.cpp:42-53) and the zero-points, 8 per uint32 along N
+    std::vector<uint32_t> packed((size_t)(K / 8) * N, 0u), qz((size_t)G * (N / 8), 0u);
+    for (int k = 0; k < K; k++)
+        for (int n = 0; n < N; n++) packed[(size_t)(k / 8) * N + n] |= (uint32_t)q[(size_t)k * N + n] << (4 * (k % 8));
+    for (int g = 0; g < G; g++)
+        for (int n = 0; n < N; n++) qz[(size_t)g * (N / 8) + n / 8] |= (uint32_t)z[(size_t)g * N + n] << (4 * (n % 8));
+    // dequantize (quantize.cpp:58-65 with zero-points): one rounding
+    cpuWdq.resize((size_t)K * N);
+    for (int k = 0; k < K; k++)
+        for (int n = 0; n < N; n++) {
+            const int g = k / GROUP_SIZE;
+            cpuWdq[(size_t)k * N + n] = cli::f32_to_f16((float)((int)q[(size_t)k * N + n] - (int)z[(size_t)g * N + n]) *
+                                                        cli::f16_to_f32(cpuScales[(size_t)g * N + n]));
+        }
+    gpuA.upload(cpuA.data());
+    gpuW.upload(packed.data());
+    gpuScales.upload(cpuScales.data());
+    gpuZ.upload(qz.data());
+}
+
+// C_ref rows (FP32 k-ascending sum over the dequantized weights), up to 64 rows
+static std::vector<int> ref_rows(int M) {
+    std::vector<int> r;
+    const int step = M > 64 ? M / 64 : 1;
+    for (int i = 0; i < M; i += step) r.push_back(i);
+    return r;
+}
+
+static void computeReference(int M, int N, int K) {
+    const auto rows = ref_rows(M);
+    cpuCref.assign(rows.size() * (size_t)N, 0);
+    std::vector<float> fw((size_t)K * N);
+    for (size_t i = 0; i < fw.size(); i++) fw[i] = cli::f16_to_f32(cpuWdq[i]);
+    cli::parallel_for(rows.size(), [&](size_t lo, size_t hi) {
+        std::vector<float> acc(N);
+        for (size_t t = lo; t < hi; t++) {
+            std::fill(acc.begin(), acc.end(), 0.0f);
+            for (int k = 0; k < K; k++) {
+                const float a = cli::f16_to_f32(cpuA[(size_t)rows[t] * K + k]);
+                const float* w = &fw[(size_t)k * N];
+                for (int n = 0; n < N; n++) acc[n] += a * w[n];
+            }
+            for (int n = 0; n < N; n++) cpuCref[t * N + n] = cli::f32_to_f16(acc[n]);
+        }
+    });
+}
+
+// w4a16.cpp:194-233: |r - ref| <= 1e-2 |ref| + 1e-3 RMS(C_ref)
+static bool compareToReference(int M, int N) {
+    const auto rows = ref_rows(M);
+    std::vector<uint16_t> got((size_t)M * N);
+    gpuC.download(got.data());
+    double ss = 0.0;
+    for (uint16_t v : cpuCref) ss += (double)cli::f16_to_f32(v) * cli::f16_to_f32(v);
+    const float atol = 1e-3f * (float)std::sqrt(ss / (double)cpuCref.size());
+    int bad = 0;
+    for (size_t t = 0; t < rows.size(); t++)
+        for (int n = 0; n < N; n++) {
+            const float r = cli::f16_to_f32(got[(size_t)rows[t] * N + n]);
+            const float ref = cli::f16_to_f32(cpuCref[t * N + n]);
+            if (std::fabs(r - ref) > 1e-2f * std::fabs(ref) + atol) {
+                if (++bad <= 8) printf("row %d col %d | GPU: %f | reference: %f\n", rows[t], n, r, ref);
+            }
+        }
+    if (bad) {
+        printf("Number of different results: %d (tolerance: 1%% + %f)\n", bad, atol);
+        cli::g_check_failed = true;
+    }
+    return bad == 0;
+}
+
+struct Metrics {
+    float time_us, gflops, gbps, pct_peak_bw, baseline_time_us, speedup_vs_baseline;
+};
+static std::map<std::vector<int>, Metrics> metrics;
+
+// minimal bytes between HBM and the CUs (w4a16.cpp:268-280 plus zero-points)
+static double bytesMoved(int M, int N, int K) {
+    return 2.0 * M * K + 4.0 * (K / 8) * N + 2.0 * (K / GROUP_SIZE) * N + 4.0 * (K / GROUP_SIZE) * (N / 8) +
+           2.0 * M * N;
+}
+
+static float median(std::vector<float> v) {
+    std::sort(v.begin(), v.end());
+    const size_t m = v.size() / 2;
+    return v.size() % 2 ? v[m] : 0.5f * (v[m - 1] + v[m]);
+}
+
+static std::vector<float> measure(W4A16Function f, const Problem& p, int runs, int warmup, bool check) {
+    std::vector<float> t;
+    for (int i = -warmup; i < runs; i++) {
+        if (check) HIP_CHECK(hipMemset(gpuC.p, 0, sizeof(uint16_t) * gpuC.n));
+        cli::flush_caches();
+        const float ms = f(p);
+        if (check) compareToReference((int)p.M, (int)p.N);
+        if (i >= 0) t.push_back(ms);
+    }
+    return t;
+}
+
+static float runner(W4A16Function impl, std::vector<int> sizes, int nb_runs, int nb_warmup_runs, bool checkResult,
+                    bool printEach, int argc, char* argv[]) {
+    const int M = sizes[0], N = sizes[1], K = sizes[2];
+    if (K % GROUP_SIZE != 0 || N % 8 != 0) {
+        fprintf(stderr, "K must be a multiple of the group size (%d) and N a multiple of 8\n", GROUP_SIZE);
+        exit(1);
+    }
+    symmetric = strcmp(argv[argc - 1], "sym") == 0;
+    nb_runs = std::max(nb_runs, 1);
+    nb_warmup_runs = std::max(nb_warmup_runs, 1);
+    gpuA.alloc((size_t)M * K);
+    gpuW.alloc((size_t)(K / 8) * N);
+    gpuScales.alloc((size_t)(K / GROUP_SIZE) * N);
+    gpuZ.alloc((size_t)(K / GROUP_SIZE) * (N / 8));
+    gpuC.alloc((size_t)M * N);
+    initProblem(M, N, K);
+    if (checkResult) computeReference(M, N, K);
+    const Problem p{gpuA.p, gpuW.p, gpuScales.p, symmetric ? nullptr : gpuZ.p, gpuC.p, (size_t)M, (size_t)N,
+                    (size_t)K};
+    const auto times = measure(impl, p, nb_runs, nb_warmup_runs, checkResult);
+    // baseline: the untuned size-based choice (the reference compares with cuBLAS fp16, w4a16.cpp:306)
+    const auto base = measure(w4a16_hip, p, nb_runs, nb_warmup_runs, false);
+    if (printEach)
+        for (int i = 0; i < nb_runs; i++)
+            printf("[internal][iter %d]: %f GFLOP/s | %.2f us (untuned: %.2f us)\n", i,
+                   2.0 * M * N * K / (times[i] * 1e6), times[i] * 1e3f, base[i] * 1e3f);
+    const float ms = median(times), bms = median(base);
+    Metrics m;
+    m.time_us = ms * 1e3f;
+    m.gflops = (float)(2.0 * M * N * K / (ms * 1e6));
+    m.gbps = (float)(bytesMoved(M, N, K) / (ms * 1e6));
+    m.pct_peak_bw = 100.0f * m.gbps / (float)cli::kHbmPeakGBs;
+    m.baseline_time_us = bms * 1e3f;
+    m.speedup_vs_baseline = bms / ms;
+    metrics[sizes] = m;
+    printf("# median: %.2f us | %.1f GB/s (%.1f%% of %.0f GB/s) | %.1f TFLOP/s (%.1f%% of FP16 MFMA) | untuned: %.2f us\n",
+           m.time_us, m.gbps, m.pct_peak_bw, cli::kHbmPeakGBs, m.gflops / 1000.0,
+           100.0 * m.gflops / 1000.0 / cli::kFp16MfmaTflops, m.baseline_time_us);
+    gpuA.free();
+    gpuW.free();
+    gpuScales.free();
+    gpuZ.free();
+    gpuC.free();
+    return m.gflops;
+}
+
+static void exportCSV(const std::map<std::vector<int>, float>& results, const std::string& filename) {
+    std::ofstream file(filename);
+    file << "M,N,K,time_us,gflops,gbps,pct_peak_bw,baseline_time_us,speedup_vs_baseline\n";
+    for (const auto& [s, g] : results) {
+        const Metrics& m = metrics.at(s);
+        file << s[0] << "," << s[1] << "," << s[2] << "," << m.time_us << "," << m.gflops << "," << m.gbps << ","
+             << m.pct_peak_bw << "," << m.baseline_time_us << "," << m.speedup_vs_baseline << "\n";
+    }
+}
+
+// ./libthom tune w4a16 M N K [runs]: every configuration of the search space,
+// validated, timed (median, caches flushed), appended to the tuning database.
+static void tune(int argc, char* argv[]) {
+    if (argc < 6) {
+        fprintf(stderr, "Usage: ./libthom tune w4a16 M N K [runs]\n");
+        exit(1);
+    }
+    const int M = atoi(argv[3]), N = atoi(argv[4]), K = atoi(argv[5]);
+    const int runs = argc > 6 ? atoi(argv[6]) : 10;
+    if (M <= 0 || N <= 0 || K <= 0 || K % GROUP_SIZE || N % 8 || runs <= 0) {
+        fprintf(stderr, "bad sizes\n");
+        exit(1);
+    }
+    gpuA.alloc((size_t)M * K);
+    gpuW.alloc((size_t)(K / 8) * N);
+    gpuScales.alloc((size_t)(K / GROUP_SIZE) * N);
+    gpuZ.alloc((size_t)(K / GROUP_SIZE) * (N / 8));
+    gpuC.alloc((size_t)M * N);
+    initProblem(M, N, K);
+    computeReference(M, N, K);
+    const Problem p{gpuA.p, gpuW.p, gpuScales.p, gpuZ.p, gpuC.p, (size_t)M, (size_t)N, (size_t)K};
+    std::vector<lthip_w4a16_config> cands;
+    if (M <= 128)
+        for (int s : {1, 2, 3, 4, 6, 8, 12, 16}) cands.push_back({1, 64, 16, s});
+    for (auto bm_bn : {std::pair<int, int>{64, 64}, {64, 128}, {128, 128}, {128, 256}, {256, 128}, {256, 256}})
+        cands.push_back({2, bm_bn.second, bm_bn.first, 1});
+    float best = 1e30f;
+    lthip_w4a16_config bestc{};
+    for (const auto& c : cands) {
+        char name[96];
+        snprintf(name, sizeof(name), "kind=%d tile_n=%d tile_m=%d split_k=%d", c.kind, c.tile_n, c.tile_m, c.split_k);
+        HIP_CHECK(hipMemset(gpuC.p, 0, sizeof(uint16_t) * gpuC.n));
+        run_cfg(p, &c);
+        if (!compareToReference(M, N)) {
+            printf("[failed] %s\n", name);
+            continue;
+        }
+        std::vector<float> t;
+        for (int r = 0; r < runs + 2; r++) {
+            cli::flush_caches();
+            const float ms = run_cfg(p, &c);
+            if (r >= 2) t.push_back(ms);
+        }
+        const float med = median(t);
+        printf("[result] %s: %.2f us\n", name, med * 1e3f);
+        tuning::append_w4a16(M, N, K, GROUP_SIZE, c, med);
+        if (med < best) {
+            best = med;
+            bestc = c;
+        }
+    }
+    printf("[best] kind=%d tile_n=%d tile_m=%d split_k=%d: %.2f us\n", bestc.kind, bestc.tile_n, bestc.tile_m,
+           bestc.split_k, best * 1e3f);
+}
+
+}  // namespace w4a16
+
+void w4a16_handler(RunMode runMode, int argc, char* argv[]) {
+    switch (runMode) {
+        case RunMode::TEST:
+            Handlers<w4a16::W4A16Function, 3, BENCH_RUNS, WARMUP_RUNS>::test_handler(argc, argv, w4a16::w4a16Impls,
+                                                                                     w4a16::runner);
+            break;
+        case RunMode::BENCH:
+            Handlers<w4a16::W4A16Function, 3, BENCH_RUNS, WARMUP_RUNS>::bench_handler(
+                argc, argv, w4a16::w4a16Impls, w4a16::runner, w4a16::SIZES, w4a16::exportCSV);
+            break;
+        case RunMode::TUNE:
+            w4a16::tune(argc, argv);
+            break;
+    }
+}
+#undef BENCH_RUNS
+#undef WARMUP_RUNS

@@ -312,20 +312,14 @@ __device__ __forceinline__ void gemm_tile(__half* smem, const __half* __restrict
             for (int r = 0; r < 16; r++) {
                 const int row = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
                 const int col = n0 + wn + 32 * j + fcol;
-                if (row < M && col < N)
-                    __hip_atomic_store(mine + (long)row * N + col, acc[i][j][r], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+                if (row < M && col < N) publish(mine + (long)row * N + col, acc[i][j][r]);
             }
     __shared__ int last_flag;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    drain();
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned t = __hip_atomic_fetch_add(out.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last_flag = (t == (unsigned)splits - 1);
-    }
+    if (threadIdx.x == 0) last_flag = (take_ticket(out.counter) == (unsigned)splits - 1);
     __syncthreads();
     if (!last_flag) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
     for (int i = 0; i < TM; i++)
 #pragma unroll
@@ -336,16 +330,14 @@ __device__ __forceinline__ void gemm_tile(__half* smem, const __half* __restrict
                 const int col = n0 + wn + 32 * j + fcol;
                 if (row < M && col < N) {
                     float sum = 0.0f;
-                    for (int sp = 0; sp < splits; sp++)
-                        sum += __hip_atomic_load(out.partial + ((long)sp * M + row) * N + col, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
+                    for (int sp = 0; sp < splits; sp++) sum += collect(out.partial + ((long)sp * M + row) * N + col);
                     __half* cp = out.C + (long)row * out.ldc + col;
                     float v = out.alpha * sum;
                     if (out.beta != 0.0f) v = fmaf(out.beta, __half2float(*cp), v);
                     *cp = __float2half_rn(v);
                 }
             }
-    if (threadIdx.x == 0) __hip_atomic_store(out.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) reset_counter(out.counter);
 }
 
 }  // namespace lt

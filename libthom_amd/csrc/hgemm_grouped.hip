@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void grouped_kernel(const GProb* __restrict__ 
     __shared__ __align__(16) __half smem[2 * (BM + BN) * BK];
     __shared__ int item_s;
     for (;;) {
-        if (threadIdx.x == 0) item_s = (int)__hip_atomic_fetch_add(&queue[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) item_s = (int)take_ticket(&queue[0]);
         __syncthreads();
         const int item = item_s;
         __syncthreads();
@@ -80,10 +80,10 @@ __global__ __launch_bounds__(256) void grouped_kernel(const GProb* __restrict__ 
     }
     // the last block out resets the queue for the next launch / graph replay
     if (threadIdx.x == 0) {
-        const unsigned done = __hip_atomic_fetch_add(&queue[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned done = take_ticket(&queue[1]);
         if (done == gridDim.x - 1) {
-            __hip_atomic_store(&queue[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&queue[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            reset_counter(&queue[0]);
+            reset_counter(&queue[1]);
         }
     }
 }

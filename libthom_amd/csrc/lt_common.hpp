@@ -57,6 +57,30 @@ __device__ __forceinline__ void st_nt(float4* p, float4 a) {
     __builtin_nontemporal_store(v, reinterpret_cast<v4f*>(p));
 }
 
+// ---- handing data from one block to another -----------------------------
+// Each XCD of an MI355X has its own L2, and the L2s are not coherent with one
+// another, so an agent-scope release/acquire fence compiles to a write-back /
+// invalidate of the whole L2 (buffer_wbl2 / buffer_inv sc1) -- per block, that
+// is microseconds.  Cross-block data instead uses relaxed agent-scope atomic
+// loads and stores (sc1: written through / read past the L2), an explicit
+// s_waitcnt so the stores have landed before the block takes its ticket, and a
+// relaxed atomic ticket counter.  The block holding the last ticket then reads
+// the published data with sc1 loads.
+__device__ __forceinline__ void publish(float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float collect(const float* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// all of this thread's memory operations have completed
+__device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
+__device__ __forceinline__ unsigned take_ticket(unsigned* counter) {
+    return __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void reset_counter(unsigned* counter) {
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // DPP move within rows of 16 lanes; every control used here reads a valid lane.
 template <int CTRL>
 __device__ __forceinline__ float dpp_f32(float v) {

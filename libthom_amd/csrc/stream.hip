@@ -127,20 +127,18 @@ __global__ __launch_bounds__(kRBlock) void sum_kernel(const float* __restrict__ 
     float s = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     s = block_sum<kRBlock>(s, red);
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[blockIdx.x], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned ticket = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (ticket == gridDim.x - 1);
+        publish(&partials[blockIdx.x], s);
+        drain();
+        is_last = (take_ticket(counter) == gridDim.x - 1);
     }
     __syncthreads();
     if (!is_last) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     float p = 0.0f;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += kRBlock)
-        p += __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += kRBlock) p += collect(&partials[b]);
     p = block_sum<kRBlock>(p, red);
     if (threadIdx.x == 0) {
         *out = p;
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        reset_counter(counter);
     }
 }
 

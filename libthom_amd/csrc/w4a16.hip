@@ -169,28 +169,26 @@ __global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restr
             for (int i = 0; i < 4; i++) {
                 const int n = n0 + 16 * kb + 4 * i;
                 if (n >= N) continue;
-                *reinterpret_cast<v4f*>(mine + (long)m * N + n) =
-                    v4f{acc[mt][0][i], acc[mt][1][i], acc[mt][2][i], acc[mt][3][i]};
+#pragma unroll
+                for (int j = 0; j < 4; j++) publish(mine + (long)m * N + n + j, acc[mt][j][i]);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        drain();
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned t = __hip_atomic_fetch_add(&counters[ntile], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last_flag = (t == (unsigned)splits - 1);
-    }
+    if (threadIdx.x == 0) last_flag = (take_ticket(&counters[ntile]) == (unsigned)splits - 1);
     __syncthreads();
     if (!last_flag) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     // the last block: 256 threads sum the 64-column x min(M, 16 MT)-row tile, 4 columns each
     const int rows = M < 16 * MT ? M : 16 * MT;
     for (int e = threadIdx.x; e < rows * (kStreamCols / 4); e += 256) {
         const int m = e / (kStreamCols / 4), n = n0 + 4 * (e % (kStreamCols / 4));
         if (n >= N) continue;
-        v4f v = v4f{0.f, 0.f, 0.f, 0.f};
+        float v[4] = {0.f, 0.f, 0.f, 0.f};
         for (int sp = 0; sp < splits; sp++) {
-            v += *reinterpret_cast<const v4f*>(partial + ((long)sp * M + m) * N + n);
+            const float* src = partial + ((long)sp * M + m) * N + n;
+#pragma unroll
+            for (int j = 0; j < 4; j++) v[j] += collect(src + j);
         }
         const uint32_t lo = (uint32_t)__half_as_ushort(__float2half_rn(v[0])) |
                             ((uint32_t)__half_as_ushort(__float2half_rn(v[1])) << 16);
@@ -198,7 +196,7 @@ __global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restr
                             ((uint32_t)__half_as_ushort(__float2half_rn(v[3])) << 16);
         *reinterpret_cast<uint2*>(C + (long)m * N + n) = make_uint2(lo, hi);
     }
-    if (threadIdx.x == 0) __hip_atomic_store(&counters[ntile], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) reset_counter(&counters[ntile]);
 }
 
 template <int BM, int BN, int BK>

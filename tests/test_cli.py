@@ -1,0 +1,79 @@
+"""The libthom CLI (libthom_amd/bin/libthom): the reference's command line
+(main.cpp, routines.h Handlers) over the gfx950 library.
+
+CPU tests cover dispatch and argument validation, which happen before any HIP
+call; GPU tests run each routine's test mode, whose CPU check makes the process
+exit non-zero on a mismatch.
+"""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "libthom_amd", "bin", "libthom")
+
+
+def run(*args, timeout=300, env=None):
+    if not os.path.exists(EXE):
+        pytest.fail("libthom CLI not built (make -C libthom_amd)")
+    return subprocess.run([EXE, *map(str, args)], capture_output=True, text=True, timeout=timeout, env=env)
+
+
+def test_usage_and_unknown_routine():
+    r = run()
+    assert r.returncode == 1 and "at least" in r.stderr
+    r = run("test", "nosuch")
+    assert r.returncode != 0 and "Invalid mode or routine" in r.stderr
+    for routine in ("reduction", "sdot", "sgemv", "dgemm", "hgemm", "w4a16"):
+        assert routine in r.stderr
+    r = run("explode", "reduction")
+    assert r.returncode != 0
+
+
+def test_missing_arguments_list_implementations():
+    r = run("test", "dgemm")
+    assert r.returncode == 1 and "<size3> <nb_iter>" in r.stderr
+    assert "hip" in r.stderr and "ozaki8" in r.stderr
+    r = run("test", "sgemv", "nope", 4, 4, 1)
+    assert r.returncode == 1 and "Invalid implementation" in r.stderr
+    r = run("test", "reduction", "hip", 100, 0)
+    assert r.returncode == 1 and "nb_iter" in r.stderr
+    r = run("tune", "hgemm", 1, 2)
+    assert r.returncode == 1 and "Usage" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args", [
+    ("reduction", "hip", 1_000_000, 3),
+    ("reduction", "hip", 1_000_003, 2),
+    ("sdot", "hip", 1_000_000, 3),
+    ("sgemv", "hip", 2048, 1024, 2, 1, "row"),
+    ("sgemv", "hip", 2000, 1000, 2, 1, "col"),
+    ("dgemm", "hip", 512, 384, 256, 2),
+    ("dgemm", "ozaki8", 256, 256, 512, 2),
+    ("dgemm", "ozaki10", 256, 256, 512, 2, 1, "scaled"),
+    ("hgemm", "hip", 1024, 512, 768, 2),
+    ("w4a16", "hip", 16, 1024, 1024, 2),
+    ("w4a16", "tiled", 192, 512, 1024, 2),
+    ("w4a16", "stream", 1, 11008, 4096, 2, 1, "sym"),
+])
+def test_cli_test_mode_passes_checks(args):
+    r = run("test", *args)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "# GFLOP/s" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cli_batched_and_tune(tmp_path):
+    r = run("test", "hgemm_batched", "persistent", 16, 2)
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = run("test", "hgemm_batched", "graph", 16, 2)
+    assert r.returncode == 0, r.stdout + r.stderr
+    env = dict(os.environ, LTHIP_TUNING_DIR=str(tmp_path))
+    r = run("tune", "hgemm", 256, 256, 512, 3, env=env)
+    assert r.returncode == 0 and "[best]" in r.stdout, r.stdout[-2000:] + r.stderr
+    rows = open(tmp_path / "hgemm.csv").read().splitlines()
+    assert rows[0] == "M,N,K,bm,bn,bk,split_k,stages,ms" and len(rows) > 10
+    r = run("test", "hgemm", "tuned", 256, 256, 512, 2, env=env)
+    assert r.returncode == 0

@@ -33,7 +33,7 @@ def single(A, B, cfg, split):
 @pytest.mark.parametrize("tile", [(128, 128, 64), (64, 64, 64), (256, 256, 64)])
 def test_grouped_matches_single(dev, orc, tile):
     dims = [(64, 64, 64), (100, 200, 72), (1000, 48, 512), (48, 1000, 4096), (512, 512, 512), (16, 16, 16),
-            (257, 129, 1000), (1024, 1024, 256), (80, 96, 2048), (0, 64, 64)]
+            (257, 136, 1000), (1024, 1024, 256), (80, 96, 2048), (0, 64, 64)]
     splits = [1, 1, 2, 8, 1, 1, 3, 1, 4, 1]
     probs = make_problems(dims, dev, 1, splits)
     cfg = HgemmConfig(*tile, 1, 2)
@@ -55,7 +55,7 @@ def test_grouped_matches_single(dev, orc, tile):
 
 
 def test_grouped_graph_replays(dev):
-    dims = [(128, 256, 512), (64, 64, 4096), (300, 100, 200), (512, 64, 1024)]
+    dims = [(128, 256, 512), (64, 64, 4096), (300, 104, 200), (512, 64, 1024)]
     probs = make_problems(dims, dev, 2, [1, 4, 1, 2])
     plan = R.GroupedHgemm(probs, HgemmConfig(128, 128, 64, 1, 2))
     side = torch.cuda.Stream(device=dev)
