@@ -180,7 +180,7 @@ typedef struct lthip_w4a16_config {
     int kind;        /* 0 = auto, 1 = small-M split-K stream, 2 = tiled MFMA */
     int tile_n;      /* kind 1: 64 columns per block (fixed); kind 2: BN */
     int tile_m;      /* kind 2: BM */
-    int split_k;     /* kind 1: K splits (>= 1) */
+    int split_k;     /* K splits (>= 1); > 1 needs the workspace */
 } lthip_w4a16_config;
 
 size_t lthip_w4a16_workspace_size(size_t M, size_t N, size_t K, const lthip_w4a16_config *cfg);
@@ -191,6 +191,24 @@ int lthip_w4a16_gemm(const uint16_t *A, const uint32_t *W_packed, const uint16_t
  * check of dequantize_w4 (quantize.cpp:58-65) on the GPU. */
 int lthip_w4_dequant(const uint32_t *W_packed, const uint16_t *scales, const uint32_t *qzeros,
                      uint16_t *W, size_t K, size_t N, int group, void *stream);
+
+/* ---- learned tuner: the cost-model MLP (no counterpart in the reference; its
+ *      tuner is the exhaustive search of hgemm.cu:258-323).  FP32, row-major:
+ *      x [B][I], W1 [H][I], b1 [H], W2 [O][H], b2 [O], hidden [B][H], y [B][O];
+ *      y = W2 relu(W1 x + b1) + b2.  loss_grad computes the masked mean squared
+ *      error against target [B][O] (mask != 0 marks measured entries) and the
+ *      gradients of all four parameters; reductions run in a fixed order. */
+int lthip_mlp_forward(const float *x, const float *W1, const float *b1, const float *W2,
+                      const float *b2, float *hidden, float *y, int B, int I, int H, int O,
+                      void *stream);
+size_t lthip_mlp_workspace_size(int B, int H, int O);
+int lthip_mlp_loss_grad(const float *x, const float *W1, const float *b1, const float *W2,
+                        const float *b2, const float *target, const uint8_t *mask, int B, int I,
+                        int H, int O, float *gW1, float *gb1, float *gW2, float *gb2, float *loss,
+                        void *ws, size_t ws_bytes, void *stream);
+/* Adam (Kingma & Ba 2015) on n parameters, bias-corrected for step >= 1. */
+int lthip_adam_step(float *params, const float *grads, float *m, float *v, int n, float lr,
+                    float beta1, float beta2, float eps, int step, void *stream);
 
 #ifdef __cplusplus
 }

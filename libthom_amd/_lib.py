@@ -88,6 +88,11 @@ _SIGS = {
     "lthip_hgemm_grouped_graph_launch": (c_int, [c_vp, c_vp]),
     "lthip_hgemm_grouped_plan_info": (c_int, [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
     "lthip_hgemm_grouped_plan_destroy": (c_int, [c_vp]),
+    "lthip_mlp_forward": (c_int, [c_vp] * 7 + [c_int] * 4 + [c_vp]),
+    "lthip_mlp_workspace_size": (c_size, [c_int, c_int, c_int]),
+    "lthip_mlp_loss_grad": (c_int, [c_vp] * 7 + [c_int] * 4 + [c_vp] * 5 + [c_vp, c_size, c_vp]),
+    "lthip_adam_step": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_float, c_int, c_vp]),
     "lthip_w4a16_workspace_size": (c_size, [c_size, c_size, c_size, ctypes.POINTER(W4A16Config)]),
     "lthip_w4a16_gemm": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_size, c_size, c_int,
                                  ctypes.POINTER(W4A16Config), c_vp, c_size, c_vp]),

@@ -202,24 +202,27 @@ __global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restr
 template <int BM, int BN, int BK>
 __global__ __launch_bounds__(256) void w4a16_tiled_kernel(const __half* __restrict__ A, BW4Params p,
                                                           __half* __restrict__ C, int M, int N, int K, int tiles_m,
-                                                          int tiles_n) {
+                                                          int tiles_n, int splits, float* __restrict__ partial,
+                                                          unsigned* __restrict__ counters) {
     __shared__ __align__(16) __half smem[2 * (BM + BN) * BK];
     // tiles in column-major order of (tm, tn) after the XCD remap: consecutive
     // blocks share the same weight columns
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int tm = t % tiles_m, tn = t / tiles_m;
+    const int split = blockIdx.y;
     TileOut o;
     o.C = C;
     o.ldc = N;
     o.alpha = 1.0f;
     o.beta = 0.0f;
-    o.partial = nullptr;
-    o.counter = nullptr;
+    o.partial = partial;
+    o.counter = counters ? counters + t : nullptr;
     const int slabs = (K + BK - 1) / BK;
-    gemm_tile<BM, BN, BK>(smem, A, (long)K, p, M, N, K, tm * BM, tn * BN, 0, slabs, 0, 1, o);
+    const int s0 = (int)((long)slabs * split / splits), s1 = (int)((long)slabs * (split + 1) / splits);
+    gemm_tile<BM, BN, BK>(smem, A, (long)K, p, M, N, K, tm * BM, tn * BN, s0, s1, split, splits, o);
 }
 
-typedef void (*tiled_fn)(const __half*, BW4Params, __half*, int, int, int, int, int);
+typedef void (*tiled_fn)(const __half*, BW4Params, __half*, int, int, int, int, int, int, float*, unsigned*);
 struct TiledEntry {
     int bm, bn;
     tiled_fn fn;
@@ -237,7 +240,7 @@ constexpr int kNumTiled = sizeof(kTiled) / sizeof(kTiled[0]);
 void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_w4a16_config* out) {
     lthip_w4a16_config c = cfg ? *cfg : lthip_w4a16_config{0, 0, 0, 0};
     if (c.kind == 0) {
-        if (M <= 128) {
+        if (M <= 64) {
             c.kind = 1;
             const long ntiles = (long)((N + kStreamCols - 1) / kStreamCols);
             const long ksteps = (long)((K / 8 + 3) / 4);
@@ -249,9 +252,14 @@ void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_
             c.tile_m = 16;
         } else {
             c.kind = 2;
-            c.tile_m = M >= 1024 ? 256 : 128;
+            c.tile_m = M >= 1024 ? 256 : (M > 128 ? 128 : 64);
             c.tile_n = (M >= 1024 && N >= 4096) ? 256 : 128;
-            c.split_k = 1;
+            // split K until the tile grid holds about two blocks per CU
+            const long tiles = (long)((M + c.tile_m - 1) / c.tile_m) * (long)((N + c.tile_n - 1) / c.tile_n);
+            const long slabs = (long)((K + 63) / 64);
+            long split = 1;
+            while (tiles * split * 2 <= 2L * cu_count() && split * 2 <= slabs / 4) split *= 2;
+            c.split_k = (int)split;
         }
     }
     if (c.split_k < 1) c.split_k = 1;
@@ -266,7 +274,7 @@ extern "C" {
 size_t lthip_w4a16_workspace_size(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg) {
     lthip_w4a16_config c;
     lt::resolve(M, N, K, cfg, &c);
-    if (c.kind != 1 || c.split_k <= 1) return 0;
+    if (c.split_k <= 1) return 0;
     return lt::kCounterBytes + sizeof(float) * (size_t)c.split_k * M * N;
 }
 
@@ -331,8 +339,24 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
         if (kTiled[i].bm == c.tile_m && kTiled[i].bn == c.tile_n) idx = i;
     if (idx < 0) return fail(LTHIP_ERR_UNSUPPORTED, "w4a16: tile %dx%d is not built", c.tile_m, c.tile_n);
     const int tiles_m = (m + c.tile_m - 1) / c.tile_m, tiles_n = (n + c.tile_n - 1) / c.tile_n;
-    hipLaunchKernelGGL(kTiled[idx].fn, dim3((unsigned)(tiles_m * tiles_n)), dim3(256), 0, s,
-                       reinterpret_cast<const __half*>(A), p, reinterpret_cast<__half*>(C), m, n, k, tiles_m, tiles_n);
+    int splits = c.split_k;
+    const int slabs = (k + 63) / 64;
+    if (splits > slabs) splits = slabs > 0 ? slabs : 1;
+    if ((size_t)tiles_m * tiles_n > kMaxCounters) splits = 1;
+    float* partial = nullptr;
+    unsigned* counters = nullptr;
+    if (splits > 1) {
+        lthip_w4a16_config c2 = c;
+        c2.split_k = splits;
+        const size_t need = lthip_w4a16_workspace_size(M, N, K, &c2);
+        if (!ws || ws_bytes < need) return fail(LTHIP_ERR_WORKSPACE, "w4a16: need %zu workspace bytes", need);
+        if (reinterpret_cast<uintptr_t>(ws) % 16) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: workspace alignment");
+        counters = reinterpret_cast<unsigned*>(ws);
+        partial = reinterpret_cast<float*>(reinterpret_cast<char*>(ws) + kCounterBytes);
+    }
+    hipLaunchKernelGGL(kTiled[idx].fn, dim3((unsigned)(tiles_m * tiles_n), (unsigned)splits), dim3(256), 0, s,
+                       reinterpret_cast<const __half*>(A), p, reinterpret_cast<__half*>(C), m, n, k, tiles_m, tiles_n,
+                       splits, partial, counters);
     LT_TRY_LAUNCH("w4a16 tiled launch");
     return LTHIP_OK;
 }

@@ -87,7 +87,8 @@ def test_w4a16_config_shapes(dev, orc, M):
 
 @pytest.mark.parametrize("cfg", [W4A16Config(1, 64, 16, 1), W4A16Config(1, 64, 16, 4), W4A16Config(1, 64, 16, 32),
                                  W4A16Config(2, 64, 64, 1), W4A16Config(2, 128, 128, 1),
-                                 W4A16Config(2, 256, 256, 1), W4A16Config(2, 128, 256, 1)])
+                                 W4A16Config(2, 256, 256, 1), W4A16Config(2, 128, 256, 1),
+                                 W4A16Config(2, 128, 64, 4), W4A16Config(2, 64, 64, 8)])
 @pytest.mark.parametrize("M", [1, 5, 16, 33, 64])
 def test_w4a16_kinds(dev, orc, cfg, M):
     N, K, group = 520, 1024, 128
