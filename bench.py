@@ -180,6 +180,8 @@ def main():
     ap.add_argument("--no-routines", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run 5 plain 256 MiB copies (known traffic, to calibrate HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md on sharding); "
@@ -215,6 +217,16 @@ def main():
     ms = start.elapsed_time(end) / args.steps
     gbs = SG_BYTES / ms / 1e6
     log(f"sgemv 16384x8192: {ms * 1e3:.1f} us/step, {gbs:.0f} GB/s")
+
+    if args.copy_probe:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        R.fill_uniform(src, 0.0, 1.0, 1234, 99)
+        for _ in range(5):
+            R.copy_f32(dst, src)
+        torch.cuda.synchronize()
+        log("copy probe: 5 x 256 MiB copied")
+        del src, dst
 
     routines = None
     if not args.no_routines:
