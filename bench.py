@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK = 78.6      # TFLOP/s dense MFMA
 FP16_PEAK = 2500.0    # TFLOP/s dense MFMA
+FP8_PEAK = 5000.0
 M_SG, N_SG = 16384, 8192
 SG_BYTES = 4 * (M_SG * N_SG + N_SG + M_SG)
 
@@ -128,16 +129,43 @@ def bench_routines(torch, R, timer, dev, quick):
         out[f"sdot_n{n}"] = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
         del xs, ys
     torch.cuda.empty_cache()
-    # dgemm native
+    # dgemm: native FP64 MFMA vs split-precision on FP8 MFMA, errors against
+    # the compensated (Dot2) product, relative to (|A||B|)_ij, on both inputs
     n = 4096
-    A = torch.from_numpy(inputs.normal_f64(n * n, 1234, inputs.S_DGEMM_A).reshape(n, n)).to(dev)
-    B = torch.from_numpy(inputs.normal_f64(n * n, 1234, inputs.S_DGEMM_B).reshape(n, n)).to(dev)
-    C = torch.empty(n, n, dtype=torch.float64, device=dev)
-    R.dgemm(A, B, C)
-    ms = timer.per_launch_ms(lambda: R.dgemm(A, B, C), max(3, it // 4))
-    tf = 2 * n ** 3 / ms / 1e9
-    out["dgemm_native_4096"] = {"ms": round(ms, 4), "TFLOP/s": round(tf, 2), "frac_fp64_mfma": round(tf / FP64_PEAK, 4)}
-    del A, B, C
+    A0 = inputs.normal_f64(n * n, 1234, inputs.S_DGEMM_A).reshape(n, n)
+    B0 = inputs.normal_f64(n * n, 1234, inputs.S_DGEMM_B).reshape(n, n)
+    rs = 2.0 ** inputs.int_uniform(n, -20, 20, 1234, inputs.S_DGEMM_RS)
+    cs = 2.0 ** inputs.int_uniform(n, -20, 20, 1234, inputs.S_DGEMM_CS)
+    study = {}
+    for label, (Ah, Bh) in (("a_normal", (A0, B0)), ("b_scaled", (A0 * rs[:, None], B0 * cs[None, :]))):
+        A = torch.from_numpy(Ah).to(dev)
+        B = torch.from_numpy(Bh).to(dev)
+        hi, lo = R.dgemm_compensated(A, B)
+        scale = R.dgemm(A.abs(), B.abs())
+        C = torch.empty(n, n, dtype=torch.float64, device=dev)
+        R.dgemm(A, B, C)
+        err = lambda Cx: float(((Cx - hi - lo).abs() / scale).max().item())
+        ms = timer.per_launch_ms(lambda: R.dgemm(A, B, C), max(3, it // 4))
+        tf = 2 * n ** 3 / ms / 1e9
+        row = {"native": {"ms": round(ms, 4), "TFLOP/s": round(tf, 2), "frac_fp64_mfma": round(tf / FP64_PEAK, 4),
+                          "max_rel_err": err(C)}}
+        for S in (4, 6, 8, 9, 10):
+            Co = torch.empty_like(C)
+            R.dgemm_ozaki(A, B, S, Co)
+            mso = timer.per_launch_ms(lambda: R.dgemm_ozaki(A, B, S, Co), max(3, it // 4))
+            tfo = 2 * n ** 3 / mso / 1e9
+            pairs = S * (S + 1) // 2
+            row[f"ozaki{S}"] = {"ms": round(mso, 4), "TFLOP/s_fp64_equiv": round(tfo, 2),
+                                "fp8_TFLOP/s": round(tfo * pairs, 1),
+                                "frac_fp8_mfma": round(tfo * pairs / FP8_PEAK, 4), "max_rel_err": err(Co)}
+        eq = [S for S in (4, 6, 8, 9, 10) if row[f"ozaki{S}"]["max_rel_err"] <= row["native"]["max_rel_err"]]
+        row["equal_error"] = ({"slices": eq[0], "emulated_TFLOP/s": row[f"ozaki{eq[0]}"]["TFLOP/s_fp64_equiv"],
+                               "native_TFLOP/s": row["native"]["TFLOP/s"]} if eq else
+                              {"slices": None, "note": "no slice count up to 10 reaches the native error"})
+        study[label] = row
+        del A, B, C, hi, lo, scale
+        torch.cuda.empty_cache()
+    out["dgemm_4096"] = study
     # hgemm
     for n in (4096, 8192):
         A = torch.empty(n, n, dtype=torch.float16, device=dev)
@@ -151,6 +179,32 @@ def bench_routines(torch, R, timer, dev, quick):
         out[f"hgemm_{n}"] = {"ms": round(ms, 4), "TFLOP/s": round(tf, 1), "frac_fp16_mfma": round(tf / FP16_PEAK, 4),
                              "config": repr(R.hgemm_default_config(n, n, n))}
         del A, B, C
+    # batched hgemm: 256 uneven problems, one persistent launch and its hipGraph
+    from libthom_amd import HgemmConfig
+    dims = R.batched_problem_dims()
+    probs = []
+    for i, (M, N, K) in enumerate(dims):
+        Ab = torch.empty(M, K, dtype=torch.float16, device=dev)
+        Bb = torch.empty(K, N, dtype=torch.float16, device=dev)
+        R.fill_uniform(Ab, -1, 1, 1234, inputs.S_BATCH_BASE + 2 * i)
+        R.fill_uniform(Bb, -1, 1, 1234, inputs.S_BATCH_BASE + 2 * i + 1)
+        tiles = ((M + 127) // 128) * ((N + 127) // 128)
+        split = (4 if K >= 2048 else 2) if (tiles < 16 and K >= 1024) else 1
+        probs.append((Ab, Bb, torch.empty(M, N, dtype=torch.float16, device=dev), split))
+    flops = sum(2 * M * N * K for M, N, K in dims)
+    plan = R.GroupedHgemm(probs, HgemmConfig(128, 128, 64, 1, 2))
+    side = torch.cuda.Stream(device=dev)
+    plan.run(side)
+    plan.capture(side)
+    with torch.cuda.stream(side):
+        plan.replay(side)
+        ms = timer.per_launch_ms(lambda: plan.replay(side), max(3, it // 2))
+    tf = flops / ms / 1e9
+    out["hgemm_batched_256"] = {"ms": round(ms, 4), "TFLOP/s": round(tf, 1), "frac_fp16_mfma": round(tf / FP16_PEAK, 4),
+                                "flop": flops, "launch": "one persistent kernel in a hipGraph", **plan.info()}
+    plan.close()
+    del probs
+    torch.cuda.empty_cache()
     # w4a16
     Nw, Kw, G = 11008, 4096, 128
     for M in (1, 16, 128, 2048):
