@@ -2,11 +2,13 @@
 // v_mfma_f32_32x32x16_f16 (the reference accumulates in FP16,
 // hgemm_better_reuse.cu:261; its CPU check accumulates in FP32, hgemm.cu:143-147).
 //
-// Tile configurations are compile-time instances of gemm_tile (gemm_core.hpp);
-// split-K is a launch-time choice.  The configuration table below is the search
-// space of the autotuner (the reference's 192-instance template list,
+// Tile configurations are compile-time instances of gemm_tile (gemm_core.hpp,
+// register-staged, 2 LDS buffers: config.stages == 2) and gemm_tile2
+// (gemm_core2.hpp, DMA ring of 3-5 LDS stages: config.stages >= 3, K % BK == 0);
+// split-K is a launch-time choice.  The two tables below are the search space
+// of the autotuner (the reference's 192-instance template list,
 // hgemm_better_reuse.cu:378-454, plays the same role).
-#include "gemm_core.hpp"
+#include "gemm_core2.hpp"
 #include "hgemm_table.hpp"
 
 namespace lt {
@@ -53,9 +55,42 @@ const int kTileDims[][3] = {LT_HGEMM_TILES(LT_HG)};
 #undef LT_HG
 constexpr int kNumTiles = sizeof(kTileDims) / sizeof(kTileDims[0]);
 
+template <class C>
+__global__ __launch_bounds__(C::THREADS) void hgemm2_kernel(const __half* __restrict__ A, long lda,
+                                                             const __half* __restrict__ B, long ldb, int M, int N,
+                                                             int K, int tiles_m, int tiles_n, int splits, TileOut out,
+                                                             unsigned* counters) {
+    __shared__ __align__(16) char smem[C::SMEM];
+    int tm, tn;
+    group_coords(xcd_remap(blockIdx.x, gridDim.x), tiles_m, tiles_n, tm, tn);
+    const int split = blockIdx.y;
+    const int slabs = K / C::BK;
+    const int s0 = (int)((long)slabs * split / splits), s1 = (int)((long)slabs * (split + 1) / splits);
+    TileOut o = out;
+    if (counters) o.counter = counters + (long)tm * tiles_n + tn;
+    gemm_tile2<C>(smem, A, lda, B, ldb, M, N, tm * C::BM, tn * C::BN, s0, s1, split, splits, o);
+}
+
+typedef void (*hgemm2_fn)(const __half*, long, const __half*, long, int, int, int, int, int, int, TileOut, unsigned*);
+struct Tile2 {
+    hgemm2_fn fn;
+    int bm, bn, bk, stages, threads;
+};
+#define LT_HG2(bm, bn, bk, st, wm, wn) \
+    {hgemm2_kernel<Core2<bm, bn, bk, st, wm, wn>>, bm, bn, bk, st, Core2<bm, bn, bk, st, wm, wn>::THREADS},
+const Tile2 kTable2[] = {LT_HGEMM2_TILES(LT_HG2)};
+#undef LT_HG2
+constexpr int kNumTiles2 = sizeof(kTable2) / sizeof(kTable2[0]);
+
 int find_tile(int bm, int bn, int bk) {
     for (int i = 0; i < kNumTiles; i++)
         if (kTileDims[i][0] == bm && kTileDims[i][1] == bn && kTileDims[i][2] == bk) return i;
+    return -1;
+}
+
+int find_tile2(int bm, int bn, int bk, int stages) {
+    for (int i = 0; i < kNumTiles2; i++)
+        if (kTable2[i].bm == bm && kTable2[i].bn == bn && kTable2[i].bk == bk && kTable2[i].stages == stages) return i;
     return -1;
 }
 
@@ -63,8 +98,13 @@ int find_tile(int bm, int bn, int bk) {
 // grid cannot fill the chip and K is long
 void default_config(long M, long N, long K, lthip_hgemm_config* c) {
     const int cus = cu_count();
-    if (M >= 2048 && N >= 2048) {
+    c->stages = 2;
+    if (M >= 2048 && N >= 2048 && K % 32 == 0) {
+        c->bm = 256; c->bn = 256; c->bk = 32; c->stages = 4;
+    } else if (M >= 2048 && N >= 2048) {
         c->bm = 256; c->bn = 256; c->bk = 64;
+    } else if (M >= 512 && N >= 512 && K % 32 == 0) {
+        c->bm = 128; c->bn = 128; c->bk = 32; c->stages = 4;
     } else if (M >= 512 && N >= 512) {
         c->bm = 128; c->bn = 128; c->bk = 64;
     } else {
@@ -74,13 +114,13 @@ void default_config(long M, long N, long K, lthip_hgemm_config* c) {
     int split = 1;
     while (split < 16 && tiles * split * 2 <= cus && K / (c->bk * split * 2) >= 4) split *= 2;
     c->split_k = split;
-    c->stages = 2;
 }
 
 }  // namespace
 
 int hgemm_check_config(const lthip_hgemm_config* c, int* idx) {
-    if (c->stages != 2 && c->stages != 0) return fail(LTHIP_ERR_UNSUPPORTED, "hgemm: only 2 LDS stages are built");
+    if (c->stages != 2 && c->stages != 0)
+        return fail(LTHIP_ERR_UNSUPPORTED, "hgemm: this entry point takes only the 2-stage tiles");
     const int t = find_tile(c->bm, c->bn, c->bk);
     if (t < 0) return fail(LTHIP_ERR_UNSUPPORTED, "hgemm: tile %dx%dx%d is not built", c->bm, c->bn, c->bk);
     if (c->split_k < 1 || c->split_k > 64) return fail(LTHIP_ERR_INVALID_ARG, "hgemm: split_k out of range");
@@ -92,15 +132,24 @@ int hgemm_check_config(const lthip_hgemm_config* c, int* idx) {
 
 extern "C" {
 
-int lthip_hgemm_config_count(void) { return lt::kNumTiles; }
+int lthip_hgemm_config_count(void) { return lt::kNumTiles + lt::kNumTiles2; }
 
 int lthip_hgemm_config_get(int index, lthip_hgemm_config* cfg) {
-    if (index < 0 || index >= lt::kNumTiles || !cfg) return lt::fail(LTHIP_ERR_INVALID_ARG, "config index");
-    cfg->bm = lt::kTileDims[index][0];
-    cfg->bn = lt::kTileDims[index][1];
-    cfg->bk = lt::kTileDims[index][2];
+    if (index < 0 || index >= lt::kNumTiles + lt::kNumTiles2 || !cfg)
+        return lt::fail(LTHIP_ERR_INVALID_ARG, "config index");
     cfg->split_k = 1;
-    cfg->stages = 2;
+    if (index < lt::kNumTiles) {
+        cfg->bm = lt::kTileDims[index][0];
+        cfg->bn = lt::kTileDims[index][1];
+        cfg->bk = lt::kTileDims[index][2];
+        cfg->stages = 2;
+    } else {
+        const lt::Tile2& t = lt::kTable2[index - lt::kNumTiles];
+        cfg->bm = t.bm;
+        cfg->bn = t.bn;
+        cfg->bk = t.bk;
+        cfg->stages = t.stages;
+    }
     return LTHIP_OK;
 }
 
@@ -128,9 +177,19 @@ int lthip_hgemm_f16(const uint16_t* A, const uint16_t* B, uint16_t* C, size_t M,
         return fail(LTHIP_ERR_INVALID_ARG, "hgemm: A and B must be 16-byte aligned");
     lthip_hgemm_config c;
     if (cfg) c = *cfg; else default_config((long)M, (long)N, (long)K, &c);
-    int t = 0;
-    int st = hgemm_check_config(&c, &t);
-    if (st != LTHIP_OK) return st;
+    int t = 0, t2 = -1;
+    if (c.stages >= 3) {
+        t2 = find_tile2(c.bm, c.bn, c.bk, c.stages);
+        if (t2 < 0)
+            return fail(LTHIP_ERR_UNSUPPORTED, "hgemm: tile %dx%dx%d with %d stages is not built", c.bm, c.bn, c.bk,
+                        c.stages);
+        if (K % c.bk != 0)
+            return fail(LTHIP_ERR_UNSUPPORTED, "hgemm: the %d-stage tiles need K %% %d == 0", c.stages, c.bk);
+        if (c.split_k < 1 || c.split_k > 64) return fail(LTHIP_ERR_INVALID_ARG, "hgemm: split_k out of range");
+    } else {
+        int st = hgemm_check_config(&c, &t);
+        if (st != LTHIP_OK) return st;
+    }
     const int m = (int)M, n = (int)N, k = (int)K;
     const int tiles_m = (m + c.bm - 1) / c.bm, tiles_n = (n + c.bn - 1) / c.bn;
     const int slabs = (k + c.bk - 1) / c.bk;
@@ -152,10 +211,16 @@ int lthip_hgemm_f16(const uint16_t* A, const uint16_t* B, uint16_t* C, size_t M,
         counters = reinterpret_cast<unsigned*>(ws);
         out.partial = reinterpret_cast<float*>(reinterpret_cast<char*>(ws) + kCounterBytes);
     }
-    BRowParams bp{reinterpret_cast<const __half*>(B), (long)n};
     dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)splits);
-    hipLaunchKernelGGL(kTable[t], grid, dim3(256), 0, as_stream(stream), reinterpret_cast<const __half*>(A), (long)k,
-                       bp, m, n, k, tiles_m, tiles_n, splits, out, counters);
+    if (t2 >= 0) {
+        hipLaunchKernelGGL(kTable2[t2].fn, grid, dim3(kTable2[t2].threads), 0, as_stream(stream),
+                           reinterpret_cast<const __half*>(A), (long)k, reinterpret_cast<const __half*>(B), (long)n, m,
+                           n, k, tiles_m, tiles_n, splits, out, counters);
+    } else {
+        BRowParams bp{reinterpret_cast<const __half*>(B), (long)n};
+        hipLaunchKernelGGL(kTable[t], grid, dim3(256), 0, as_stream(stream), reinterpret_cast<const __half*>(A),
+                           (long)k, bp, m, n, k, tiles_m, tiles_n, splits, out, counters);
+    }
     LT_TRY_LAUNCH("hgemm launch");
     return LTHIP_OK;
 }

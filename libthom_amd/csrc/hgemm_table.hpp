@@ -10,3 +10,16 @@
     X(128, 256, 64)       \
     X(256, 128, 64)       \
     X(256, 256, 64)
+
+// Pipelined tiles (BM, BN, BK, STAGES, wave grid WGM x WGN): both operands
+// reach LDS by DMA through a ring of STAGES slabs.  Requires K % BK == 0.
+#define LT_HGEMM2_TILES(X)    \
+    X(256, 256, 32, 4, 2, 4)  \
+    X(256, 256, 32, 3, 2, 2)  \
+    X(256, 256, 32, 5, 2, 2)  \
+    X(256, 128, 64, 3, 2, 2)  \
+    X(128, 256, 64, 3, 2, 2)  \
+    X(256, 128, 32, 4, 2, 2)  \
+    X(128, 256, 32, 4, 2, 2)  \
+    X(128, 128, 64, 3, 2, 2)  \
+    X(128, 128, 32, 4, 2, 2)

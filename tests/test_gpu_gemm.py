@@ -73,34 +73,62 @@ def hgemm_check(got, ref, A, B):
 
 
 def hgemm_cfgs():
+    """Every compiled tile: the 2-stage register-staged ones and the DMA-ring ones."""
     from libthom_amd._lib import HgemmConfig
-    tiles = [(64, 64, 64), (64, 128, 64), (128, 64, 64), (128, 128, 32), (128, 128, 64), (128, 256, 64),
-             (256, 128, 64), (256, 256, 64)]
-    return [HgemmConfig(bm, bn, bk, 1, 2) for bm, bn, bk in tiles]
+    tiles = [(64, 64, 64, 2), (64, 128, 64, 2), (128, 64, 64, 2), (128, 128, 32, 2), (128, 128, 64, 2),
+             (128, 256, 64, 2), (256, 128, 64, 2), (256, 256, 64, 2),
+             (256, 256, 32, 4), (256, 256, 32, 3), (256, 256, 32, 5), (256, 128, 64, 3), (128, 256, 64, 3),
+             (256, 128, 32, 4), (128, 256, 32, 4), (128, 128, 64, 3), (128, 128, 32, 4)]
+    return [HgemmConfig(bm, bn, bk, 1, st) for bm, bn, bk, st in tiles]
 
 
-@pytest.mark.parametrize("ci", range(8))
-@pytest.mark.parametrize("M,N,K", [(256, 256, 256), (96, 200, 72), (520, 264, 1032)])
+def test_hgemm_cfgs_cover_library():
+    got = sorted((c.bm, c.bn, c.bk, c.stages) for c in R.hgemm_configs())
+    assert got == sorted((c.bm, c.bn, c.bk, c.stages) for c in hgemm_cfgs())
+
+
+@pytest.mark.parametrize("ci", range(17))
+@pytest.mark.parametrize("M,N,K", [(256, 256, 256), (96, 200, 72), (520, 264, 1032), (96, 200, 64), (33, 8, 32),
+                                   (1000, 1032, 1024)])
 def test_hgemm_tiles_small_integers_exact(dev, orc, ci, M, N, K):
+    from libthom_amd._lib import LthipError
     cfg = hgemm_cfgs()[ci]
     rng = np.random.default_rng(ci * 100 + M)
     A = rng.integers(-2, 3, (M, K)).astype(np.float16)
     B = rng.integers(-2, 3, (K, N)).astype(np.float16)
+    if cfg.stages >= 3 and K % cfg.bk:
+        with pytest.raises(LthipError):
+            R.hgemm(t(A, dev), t(B, dev), config=cfg)
+        return
     got = R.hgemm(t(A, dev), t(B, dev), config=cfg).cpu().numpy()
     assert np.array_equal(got, orc.hgemm(A, B)), cfg
 
 
+@pytest.mark.parametrize("tile", [(128, 128, 64, 2), (256, 256, 32, 4), (128, 128, 32, 4), (256, 128, 64, 3)])
 @pytest.mark.parametrize("split", [1, 2, 3, 8])
 @pytest.mark.parametrize("M,N,K", [(512, 512, 512), (64, 128, 4096), (1024, 1024, 1024)])
-def test_hgemm_random_split_k(dev, orc, split, M, N, K):
+def test_hgemm_random_split_k(dev, orc, tile, split, M, N, K):
     from libthom_amd._lib import HgemmConfig
     rng = np.random.default_rng(split + M)
     A = rng.uniform(-1, 1, (M, K)).astype(np.float16)
     B = rng.uniform(-1, 1, (K, N)).astype(np.float16)
     C0 = rng.uniform(-1, 1, (M, N)).astype(np.float16)
-    cfg = HgemmConfig(128, 128, 64, split, 2)
+    bm, bn, bk, st = tile
+    cfg = HgemmConfig(bm, bn, bk, split, st)
     got = R.hgemm(t(A, dev), t(B, dev), t(C0, dev), 1.0, 0.5, config=cfg).cpu().numpy()
     hgemm_check(got, orc.hgemm(A, B, C0, 1.0, 0.5), A, B)
+
+
+@pytest.mark.parametrize("stages", [2, 4])
+def test_hgemm_empty_k_scales_c(dev, orc, stages):
+    from libthom_amd._lib import HgemmConfig
+    M, N = 520, 512
+    A = np.zeros((M, 0), np.float16)
+    B = np.zeros((0, N), np.float16)
+    C0 = np.random.default_rng(9).uniform(-1, 1, (M, N)).astype(np.float16)
+    cfg = HgemmConfig(128, 128, 32, 1, stages)
+    got = R.hgemm(t(A, dev), t(B, dev), t(C0, dev), 1.0, 0.5, config=cfg).cpu().numpy()
+    assert np.array_equal(got, orc.hgemm(A, B, C0, 1.0, 0.5))
 
 
 def test_hgemm_default_config_large(dev, orc):
