@@ -180,19 +180,12 @@ def bench_routines(torch, R, timer, dev, quick):
                              "config": repr(R.hgemm_default_config(n, n, n))}
         del A, B, C
     # batched hgemm: 256 uneven problems, one persistent launch and its hipGraph
-    from libthom_amd import HgemmConfig
+    from libthom_amd import tuner
     dims = R.batched_problem_dims()
-    probs = []
-    for i, (M, N, K) in enumerate(dims):
-        Ab = torch.empty(M, K, dtype=torch.float16, device=dev)
-        Bb = torch.empty(K, N, dtype=torch.float16, device=dev)
-        R.fill_uniform(Ab, -1, 1, 1234, inputs.S_BATCH_BASE + 2 * i)
-        R.fill_uniform(Bb, -1, 1, 1234, inputs.S_BATCH_BASE + 2 * i + 1)
-        tiles = ((M + 127) // 128) * ((N + 127) // 128)
-        split = (4 if K >= 2048 else 2) if (tiles < 16 and K >= 1024) else 1
-        probs.append((Ab, Bb, torch.empty(M, N, dtype=torch.float16, device=dev), split))
+    bcfg, policy = tuner.best_batched(len(dims))
+    probs, _ = tuner.batched_problems(dims, dev, policy, bcfg)
     flops = sum(2 * M * N * K for M, N, K in dims)
-    plan = R.GroupedHgemm(probs, HgemmConfig(128, 128, 64, 1, 2))
+    plan = R.GroupedHgemm(probs, bcfg)
     side = torch.cuda.Stream(device=dev)
     plan.run(side)
     plan.capture(side)
@@ -201,7 +194,8 @@ def bench_routines(torch, R, timer, dev, quick):
         ms = timer.per_launch_ms(lambda: plan.replay(side), max(3, it // 2))
     tf = flops / ms / 1e9
     out["hgemm_batched_256"] = {"ms": round(ms, 4), "TFLOP/s": round(tf, 1), "frac_fp16_mfma": round(tf / FP16_PEAK, 4),
-                                "flop": flops, "launch": "one persistent kernel in a hipGraph", **plan.info()}
+                                "flop": flops, "launch": "one persistent kernel in a hipGraph",
+                                "config": repr(bcfg), "split_policy": policy, **plan.info()}
     plan.close()
     del probs
     torch.cuda.empty_cache()

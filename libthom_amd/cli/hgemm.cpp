@@ -304,6 +304,11 @@ void hgemm_batched_handler(RunMode runMode, int argc, char* argv[]) {
         fprintf(stderr, "count and nb_iter must be positive\n");
         exit(1);
     }
+    // tile and split-K policy: the tuned choice (tuning/batched.csv), else
+    // the 256x256x32 4-stage tile without split-K
+    lthip_hgemm_config cfg{256, 256, 32, 1, 4};
+    int policy = 0;
+    tuning::lookup_batched(count, &cfg, &policy);
     Batch b;
     const auto dims = dims_of(count);
     for (int i = 0; i < count; i++) {
@@ -315,14 +320,13 @@ void hgemm_batched_handler(RunMode runMode, int argc, char* argv[]) {
         LTHIP_CHECK(lthip_fill_uniform_f16(A->p, A->n, -1.0, 1.0, cli::kSeed, 1000 + 2 * i, cli::stream()));
         LTHIP_CHECK(lthip_fill_uniform_f16(B->p, B->n, -1.0, 1.0, cli::kSeed, 1001 + 2 * i, cli::stream()));
         lthip_hgemm_problem p{A->p, B->p, C->p, M, N, K, 0, 0, 0, 1};
-        // split-K for problems whose tile grid is small next to their K
-        const long tiles = (long)((M + 127) / 128) * ((N + 127) / 128);
-        if (tiles < 16 && K >= 1024) p.split_k = K >= 2048 ? 4 : 2;
+        // policy 1: split-K for problems whose tile grid is small next to their K
+        const long tiles = (long)((M + cfg.bm - 1) / cfg.bm) * ((N + cfg.bn - 1) / cfg.bn);
+        if (policy == 1 && tiles < 16 && K >= 1024) p.split_k = K >= 2048 ? 4 : 2;
         b.probs.push_back(p);
         b.flops += 2.0 * M * N * K;
     }
-    lthip_hgemm_config cfg{128, 128, 64, 1, 2};
-    tuning::lookup_batched(count, &cfg);
+    printf("[config] BM=%d BN=%d BK=%d stages=%d split policy %d\n", cfg.bm, cfg.bn, cfg.bk, cfg.stages, policy);
     const float ms = run_batch(b, cfg, iters, graph);
     if (runMode == RunMode::TEST) {
         // every problem against the single-problem kernel with the same tile/split

@@ -91,13 +91,14 @@ void append_w4a16(int M, int N, int K, int group, const lthip_w4a16_config& c, f
     append_row("w4a16.csv", "M,N,K,group,kind,tile_n,tile_m,split_k,ms", row);
 }
 
-bool lookup_batched(int count, lthip_hgemm_config* c) {
+bool lookup_batched(int count, lthip_hgemm_config* c, int* policy) {
     double best = std::numeric_limits<double>::infinity();
     for (const auto& r : read_rows("batched.csv")) {
-        if (r.size() < 5 || (int)r[0] != count) continue;
-        if (r[4] < best) {
-            best = r[4];
-            *c = lthip_hgemm_config{(int)r[1], (int)r[2], (int)r[3], 1, 2};
+        if (r.size() < 7 || (int)r[0] != count) continue;
+        if (r[6] < best) {
+            best = r[6];
+            *c = lthip_hgemm_config{(int)r[1], (int)r[2], (int)r[3], 1, (int)r[4]};
+            *policy = (int)r[5];
         }
     }
     return best < std::numeric_limits<double>::infinity();

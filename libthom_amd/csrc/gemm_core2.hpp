@@ -21,7 +21,7 @@
 // Block = WGM x WGN waves, each owning a (BM/WGM) x (BN/WGN) sub-tile of
 // v_mfma_f32_32x32x16_f16 tiles with FP32 accumulators.
 //
-// Requirements (checked by the caller): K % BK == 0, N % 8 == 0, lda and ldb
+// Requirements (checked by the caller): K % 8 == 0, N % 8 == 0, lda and ldb
 // multiples of 8, A and B 16-byte aligned.  Rows past M and columns past N are
 // read from clamped addresses and never stored.
 #pragma once
@@ -198,13 +198,47 @@ struct Core2Frags {
     }
 };
 
-// Computes the BM x BN tile of C at (m0, n0) over K-slabs [s0, s1); with
+// One slab at k0 into stage 0, every address clamped into the matrices (rows
+// to M-1 and K-1, columns to N-8, k chunks to K-8): the partial last slab.
+// What lands beyond K is masked out by the caller.
+template <class C>
+__device__ __forceinline__ void core2_issue_clamped(char* smem, const __half* __restrict__ A, long lda,
+                                                    const __half* __restrict__ B, long ldb, int M, int N, int K,
+                                                    int m0, int n0, int k0) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+#pragma unroll
+    for (int i = 0; i < C::A_INSTR; i++) {
+        const int inst = wave * C::A_INSTR + i;
+        const int row = inst * C::A_ROWS_PER + lane / C::KC;
+        const int kc = (lane % C::KC) ^ swzA2<C::BK>(row);
+        int gm = m0 + row, gk = k0 + kc * 8;
+        gm = gm < M ? gm : M - 1;
+        gk = gk < K ? gk : K - 8;
+        __builtin_amdgcn_global_load_lds(A + (long)gm * lda + gk, (lds_void_t*)(lds_char_t*)(smem + inst * 1024), 16,
+                                         0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < C::B_INSTR; i++) {
+        const int inst = wave * C::B_INSTR + i;
+        const int row = inst * C::B_ROWS_PER + lane / C::NC;
+        const int nc = (lane % C::NC) ^ swzB2(row);
+        int gn = n0 + nc * 8, gk = k0 + row;
+        gn = gn < N ? gn : N - 8;
+        gk = gk < K ? gk : K - 1;
+        __builtin_amdgcn_global_load_lds(B + (long)gk * ldb + gn,
+                                         (lds_void_t*)(lds_char_t*)(smem + C::A_BYTES + inst * 1024), 16, 0, 0);
+    }
+}
+
+// Computes the BM x BN tile of C at (m0, n0) over K-slabs [s0, s1) of BK
+// (the last slab of the matrix may be partial: K % 8 == 0 suffices); with
 // splits > 1 the partial sums go through out.partial and the last split to
 // finish sums them in split order (same protocol as gemm_tile).
 template <class C>
 __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict__ A, long lda,
-                                           const __half* __restrict__ B, long ldb, int M, int N, int m0, int n0,
-                                           int s0, int s1, int split, int splits, const TileOut& out) {
+                                           const __half* __restrict__ B, long ldb, int M, int N, int K, int m0,
+                                           int n0, int s0, int s1, int split, int splits, const TileOut& out) {
     constexpr int TM = C::TM, TN = C::TN, BK = C::BK, STAGES = C::STAGES, KSTEPS = BK / 16;
     constexpr int MFMAS = TM * TN, READS = TM + 2 * TN;
     const int lane = threadIdx.x & 63;
@@ -219,8 +253,12 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
 
-    const int nslabs = s1 - s0;
-    if (nslabs > 0) {   // K == 0 leaves the accumulators at zero
+    // full slabs go through the ring; a partial last slab (K % BK != 0) is
+    // multiplied afterwards with the k beyond K masked out
+    const int rem = K - (s1 - 1) * BK;   // valid k in the last slab of the range
+    const bool partial = s1 > s0 && rem < BK;
+    const int nslabs = s1 - s0 - (partial ? 1 : 0);
+    if (nslabs > 0) {
         Core2Src<C> src;
         src.init(A, lda, B, ldb, M, N, m0, n0, s0 * BK);
         wait_vmcnt<0>();   // nothing older may be counted against the ring below
@@ -288,6 +326,29 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
             }
         }
         wait_vmcnt<0>();   // the redundant tail DMA must land before the LDS is reused
+    }
+    if (partial) {
+        const int k0 = (s1 - 1) * BK;
+        ring_barrier<0>();   // every wave is done with the ring
+        core2_issue_clamped<C>(smem, A, lda, B, ldb, M, N, K, m0, n0, k0);
+        ring_barrier<0>();
+        const int kl = 8 * (lane >> 5);   // first k of this lane's 8 in a k-step
+#pragma unroll
+        for (int kk = 0; kk < KSTEPS; kk++) {
+            Core2Frags<C> f;
+            f.load(smem, smem + C::A_BYTES, kk, wm, wn);
+            if (16 * kk + kl >= rem) {
+#pragma unroll
+                for (int i = 0; i < TM; i++) f.a[i] = v8h{};
+#pragma unroll
+                for (int j = 0; j < TN; j++) f.b[j] = v8h{};
+            }
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+        }
     }
     __syncthreads();   // the caller may refill the ring for its next tile
 

@@ -4,7 +4,7 @@
 //
 // Tile configurations are compile-time instances of gemm_tile (gemm_core.hpp,
 // register-staged, 2 LDS buffers: config.stages == 2) and gemm_tile2
-// (gemm_core2.hpp, DMA ring of 3-5 LDS stages: config.stages >= 3, K % BK == 0);
+// (gemm_core2.hpp, DMA ring of 3-5 LDS stages: config.stages >= 3);
 // split-K is a launch-time choice.  The two tables below are the search space
 // of the autotuner (the reference's 192-instance template list,
 // hgemm_better_reuse.cu:378-454, plays the same role).
@@ -64,11 +64,11 @@ __global__ __launch_bounds__(C::THREADS) void hgemm2_kernel(const __half* __rest
     int tm, tn;
     group_coords(xcd_remap(blockIdx.x, gridDim.x), tiles_m, tiles_n, tm, tn);
     const int split = blockIdx.y;
-    const int slabs = K / C::BK;
+    const int slabs = (K + C::BK - 1) / C::BK;
     const int s0 = (int)((long)slabs * split / splits), s1 = (int)((long)slabs * (split + 1) / splits);
     TileOut o = out;
     if (counters) o.counter = counters + (long)tm * tiles_n + tn;
-    gemm_tile2<C>(smem, A, lda, B, ldb, M, N, tm * C::BM, tn * C::BN, s0, s1, split, splits, o);
+    gemm_tile2<C>(smem, A, lda, B, ldb, M, N, K, tm * C::BM, tn * C::BN, s0, s1, split, splits, o);
 }
 
 typedef void (*hgemm2_fn)(const __half*, long, const __half*, long, int, int, int, int, int, int, TileOut, unsigned*);
@@ -99,14 +99,10 @@ int find_tile2(int bm, int bn, int bk, int stages) {
 void default_config(long M, long N, long K, lthip_hgemm_config* c) {
     const int cus = cu_count();
     c->stages = 2;
-    if (M >= 2048 && N >= 2048 && K % 32 == 0) {
+    if (M >= 2048 && N >= 2048) {
         c->bm = 256; c->bn = 256; c->bk = 32; c->stages = 4;
-    } else if (M >= 2048 && N >= 2048) {
-        c->bm = 256; c->bn = 256; c->bk = 64;
-    } else if (M >= 512 && N >= 512 && K % 32 == 0) {
-        c->bm = 128; c->bn = 128; c->bk = 32; c->stages = 4;
     } else if (M >= 512 && N >= 512) {
-        c->bm = 128; c->bn = 128; c->bk = 64;
+        c->bm = 128; c->bn = 128; c->bk = 32; c->stages = 4;
     } else {
         c->bm = 64; c->bn = 64; c->bk = 64;
     }
@@ -183,8 +179,6 @@ int lthip_hgemm_f16(const uint16_t* A, const uint16_t* B, uint16_t* C, size_t M,
         if (t2 < 0)
             return fail(LTHIP_ERR_UNSUPPORTED, "hgemm: tile %dx%dx%d with %d stages is not built", c.bm, c.bn, c.bk,
                         c.stages);
-        if (K % c.bk != 0)
-            return fail(LTHIP_ERR_UNSUPPORTED, "hgemm: the %d-stage tiles need K %% %d == 0", c.stages, c.bk);
         if (c.split_k < 1 || c.split_k > 64) return fail(LTHIP_ERR_INVALID_ARG, "hgemm: split_k out of range");
     } else {
         int st = hgemm_check_config(&c, &t);

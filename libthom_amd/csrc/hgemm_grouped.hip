@@ -19,7 +19,7 @@
 #include <new>
 #include <vector>
 
-#include "gemm_core.hpp"
+#include "gemm_core2.hpp"
 #include "hgemm_table.hpp"
 
 namespace lt {
@@ -88,7 +88,67 @@ __global__ __launch_bounds__(256) void grouped_kernel(const GProb* __restrict__ 
     }
 }
 
+// Same queue protocol with the DMA-ring tile (config.stages >= 3).  The work
+// item index travels through the first word of the ring, which is idle
+// between tiles.
+template <class C>
+__global__ __launch_bounds__(C::THREADS) void grouped2_kernel(const GProb* __restrict__ probs, int count, int total,
+                                                               unsigned* __restrict__ queue, float alpha, float beta) {
+    __shared__ __align__(16) char smem[C::SMEM];
+    int* item_s = reinterpret_cast<int*>(smem);
+    for (;;) {
+        if (threadIdx.x == 0) *item_s = (int)take_ticket(&queue[0]);
+        __syncthreads();
+        const int item = *item_s;
+        __syncthreads();
+        if (item >= total) break;
+        int lo = 0, hi = count - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (probs[mid].work_begin <= item) lo = mid; else hi = mid - 1;
+        }
+        const GProb& p = probs[lo];
+        const int local = item - p.work_begin;
+        const int tile = local / p.splits, split = local % p.splits;
+        constexpr int G = 8;
+        const int per_group = G * p.tiles_n;
+        const int first_m = (tile / per_group) * G;
+        int gsize = p.tiles_m - first_m;
+        if (gsize > G) gsize = G;
+        const int tm = first_m + (tile % per_group) % gsize, tn = (tile % per_group) / gsize;
+        const int slabs = (p.K + C::BK - 1) / C::BK;
+        const int s0 = (int)((long)slabs * split / p.splits), s1 = (int)((long)slabs * (split + 1) / p.splits);
+        TileOut o;
+        o.C = p.C;
+        o.ldc = p.ldc;
+        o.alpha = alpha;
+        o.beta = beta;
+        o.partial = p.partial;
+        o.counter = p.counters ? p.counters + tile : nullptr;
+        gemm_tile2<C>(smem, p.A, (long)p.lda, p.B, (long)p.ldb, p.M, p.N, p.K, tm * C::BM, tn * C::BN, s0, s1, split,
+                      p.splits, o);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const unsigned done = take_ticket(&queue[1]);
+        if (done == gridDim.x - 1) {
+            reset_counter(&queue[0]);
+            reset_counter(&queue[1]);
+        }
+    }
+}
+
 typedef void (*grouped_fn)(const GProb*, int, int, unsigned*, float, float);
+struct GTile2 {
+    grouped_fn fn;
+    int bm, bn, bk, stages, threads, smem;
+};
+#define LT_HG2(bm, bn, bk, st, wm, wn)                                                   \
+    {grouped2_kernel<Core2<bm, bn, bk, st, wm, wn>>, bm, bn, bk, st,                     \
+     Core2<bm, bn, bk, st, wm, wn>::THREADS, Core2<bm, bn, bk, st, wm, wn>::SMEM},
+const GTile2 kGTable2[] = {LT_HGEMM2_TILES(LT_HG2)};
+#undef LT_HG2
+constexpr int kNumGTiles2 = sizeof(kGTable2) / sizeof(kGTable2[0]);
 template <int BM, int BN, int BK>
 constexpr grouped_fn gfn_of() {
     return grouped_kernel<BM, BN, BK>;
@@ -97,9 +157,8 @@ constexpr grouped_fn gfn_of() {
 const grouped_fn kGTable[] = {LT_HGEMM_TILES(LT_HG)};
 #undef LT_HG
 
-// blocks of one tile shape that fit on a CU at once (LDS and 256 threads)
-int resident_per_cu(const lthip_hgemm_config& c) {
-    const int lds = 2 * (c.bm + c.bn) * c.bk * 2;
+// blocks that fit on a CU at once (LDS bytes per block)
+int resident_per_cu(int lds) {
     int by_lds = 163840 / (lds + 64);
     if (by_lds < 1) by_lds = 1;
     return by_lds < 8 ? by_lds : 8;
@@ -111,6 +170,8 @@ size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 }  // namespace lt
 
 struct lthip_grouped_plan {
+    lt::grouped_fn fn;
+    int threads;
     int tile_index;
     int count;
     int total;
@@ -193,9 +254,29 @@ int lthip_hgemm_grouped_plan_create(const lthip_hgemm_problem* probs, int count,
     if (!plan) return fail(LTHIP_ERR_INVALID_ARG, "grouped: null plan pointer");
     *plan = nullptr;
     if (!probs || count <= 0 || !cfg) return fail(LTHIP_ERR_INVALID_ARG, "grouped: no problems or config");
-    int t = 0;
-    int st = hgemm_check_config(cfg, &t);
-    if (st != LTHIP_OK) return st;
+    int t = 0, st = LTHIP_OK;
+    grouped_fn fn = nullptr;
+    int threads = 256, lds = 0;
+    if (cfg->stages >= 3) {
+        for (int i = 0; i < kNumGTiles2 && !fn; i++) {
+            const GTile2& g = kGTable2[i];
+            if (g.bm == cfg->bm && g.bn == cfg->bn && g.bk == cfg->bk && g.stages == cfg->stages) {
+                fn = g.fn;
+                threads = g.threads;
+                lds = g.smem;
+                t = i;
+            }
+        }
+        if (!fn)
+            return fail(LTHIP_ERR_UNSUPPORTED, "grouped: tile %dx%dx%d with %d stages is not built", cfg->bm, cfg->bn,
+                        cfg->bk, cfg->stages);
+        if (cfg->split_k < 1 || cfg->split_k > 64) return fail(LTHIP_ERR_INVALID_ARG, "grouped: split_k out of range");
+    } else {
+        st = hgemm_check_config(cfg, &t);
+        if (st != LTHIP_OK) return st;
+        fn = kGTable[t];
+        lds = 2 * (cfg->bm + cfg->bn) * cfg->bk * 2;
+    }
     for (int i = 0; i < count; i++) {
         const lthip_hgemm_problem& q = probs[i];
         if (q.M < 0 || q.N < 0 || q.K < 0) return fail(LTHIP_ERR_INVALID_ARG, "grouped: problem %d negative size", i);
@@ -230,10 +311,12 @@ int lthip_hgemm_grouped_plan_create(const lthip_hgemm_problem* probs, int count,
     if (st != LTHIP_OK) return st;
     lthip_grouped_plan* p = new (std::nothrow) lthip_grouped_plan;
     if (!p) return fail(LTHIP_ERR_INVALID_ARG, "grouped: out of host memory");
+    p->fn = fn;
+    p->threads = threads;
     p->tile_index = t;
     p->count = count;
     p->total = work;
-    const int cap = cu_count() * resident_per_cu(*cfg);
+    const int cap = cu_count() * resident_per_cu(lds);
     p->grid = work < cap ? (work > 0 ? work : 1) : cap;
     p->alpha = __half2float(__ushort_as_half(alpha));
     p->beta = __half2float(__ushort_as_half(beta));
@@ -248,7 +331,7 @@ int lthip_hgemm_grouped_plan_create(const lthip_hgemm_problem* probs, int count,
 int lthip_hgemm_grouped_run(lthip_grouped_plan* plan, void* stream) {
     using namespace lt;
     if (!plan) return fail(LTHIP_ERR_INVALID_ARG, "grouped: null plan");
-    hipLaunchKernelGGL(kGTable[plan->tile_index], dim3((unsigned)plan->grid), dim3(256), 0, as_stream(stream),
+    hipLaunchKernelGGL(plan->fn, dim3((unsigned)plan->grid), dim3((unsigned)plan->threads), 0, as_stream(stream),
                        plan->table, plan->count, plan->total, plan->queue, plan->alpha, plan->beta);
     LT_TRY_LAUNCH("grouped hgemm launch");
     return LTHIP_OK;
@@ -269,7 +352,7 @@ int lthip_hgemm_grouped_capture(lthip_grouped_plan* plan, void* stream) {
     }
     int st = check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "grouped: begin capture");
     if (st != LTHIP_OK) return st;
-    hipLaunchKernelGGL(kGTable[plan->tile_index], dim3((unsigned)plan->grid), dim3(256), 0, s, plan->table,
+    hipLaunchKernelGGL(plan->fn, dim3((unsigned)plan->grid), dim3((unsigned)plan->threads), 0, s, plan->table,
                        plan->count, plan->total, plan->queue, plan->alpha, plan->beta);
     hipError_t le = hipGetLastError();
     hipGraph_t g = nullptr;

@@ -12,7 +12,7 @@
     X(256, 256, 64)
 
 // Pipelined tiles (BM, BN, BK, STAGES, wave grid WGM x WGN): both operands
-// reach LDS by DMA through a ring of STAGES slabs.  Requires K % BK == 0.
+// reach LDS by DMA through a ring of STAGES slabs.
 #define LT_HGEMM2_TILES(X)    \
     X(256, 256, 32, 4, 2, 4)  \
     X(256, 256, 32, 3, 2, 2)  \

@@ -15,7 +15,13 @@ winners were copied by hand into a size-keyed table (hgemm_better_reuse.cu:
 357-376); "the method without the network" here is the library's built-in
 size-based choice (lthip_w4a16_gemm with a null config).
 
+Batched hgemm: one tile shape (tile, LDS stages) for the whole batch plus a
+split-K policy per problem, chosen by timing every candidate on the batch
+(graph replays) -> tuning/batched.csv, read back by best_batched() and by the
+CLI's hgemm_batched routine.
+
 Usage (on a GPU box):
+  python -m libthom_amd.tuner batched --count 256
   python -m libthom_amd.tuner sweep --shapes 2000 --out libthom_amd/tuning/w4a16_sweep.csv
   python -m libthom_amd.tuner train --db libthom_amd/tuning/w4a16_sweep.csv --out libthom_amd/tuning/w4a16_mlp.json
   python -m libthom_amd.tuner eval --db ... --model ...
@@ -355,6 +361,109 @@ def read_default(path):
     return out
 
 
+BATCHED_COLUMNS = ["count", "bm", "bn", "bk", "stages", "policy", "ms"]
+# split-K policies for the batched hgemm: 0 none; 1 split problems whose tile
+# grid is small next to their K (2 splits from K >= 1024, 4 from K >= 2048)
+BATCHED_POLICIES = (0, 1)
+
+
+def batched_split(policy: int, M: int, N: int, K: int, bm: int, bn: int) -> int:
+    if policy == 0:
+        return 1
+    tiles = ((M + bm - 1) // bm) * ((N + bn - 1) // bn)
+    if tiles < 16 and K >= 1024:
+        return 4 if K >= 2048 else 2
+    return 1
+
+
+def best_batched(count: int = 256, path: Optional[str] = None):
+    """(HgemmConfig, policy) of the fastest row for `count` problems in the
+    batched database, or the built-in choice when there is none."""
+    from ._lib import HgemmConfig
+    path = path or os.path.join(TUNING_DIR, "batched.csv")
+    best = None
+    if os.path.exists(path):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if int(r["count"]) != count:
+                    continue
+                ms = float(r["ms"])
+                if best is None or ms < best[0]:
+                    best = (ms, int(r["bm"]), int(r["bn"]), int(r["bk"]), int(r["stages"]), int(r["policy"]))
+    if best is None:
+        return HgemmConfig(256, 256, 32, 1, 4), 0
+    return HgemmConfig(best[1], best[2], best[3], 1, best[4]), best[5]
+
+
+def batched_problems(dims, device, split_policy: int, cfg, tensors=None):
+    """Problem list for GroupedHgemm on the seeded batched inputs (U[-1,1),
+    stream ids S_BATCH_BASE + 2i and + 2i + 1); `tensors` reuses buffers."""
+    import torch
+    from . import inputs, routines as R
+    if tensors is None:
+        tensors = []
+        for i, (M, N, K) in enumerate(dims):
+            A = torch.empty(M, K, dtype=torch.float16, device=device)
+            B = torch.empty(K, N, dtype=torch.float16, device=device)
+            R.fill_uniform(A, -1, 1, 1234, inputs.S_BATCH_BASE + 2 * i)
+            R.fill_uniform(B, -1, 1, 1234, inputs.S_BATCH_BASE + 2 * i + 1)
+            tensors.append((A, B, torch.empty(M, N, dtype=torch.float16, device=device)))
+    probs = [(A, B, C, batched_split(split_policy, M, N, K, cfg.bm, cfg.bn))
+             for (A, B, C), (M, N, K) in zip(tensors, dims)]
+    return probs, tensors
+
+
+def time_batched(probs, cfg, device, replays: int = 10) -> float:
+    """Median ms per graph replay of the grouped launch."""
+    import torch
+    from . import routines as R
+    plan = R.GroupedHgemm(probs, cfg)
+    side = torch.cuda.Stream(device=device)
+    try:
+        plan.run(side)
+        plan.capture(side)
+        times = []
+        with torch.cuda.stream(side):
+            plan.replay(side)
+            for _ in range(replays):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(side)
+                plan.replay(side)
+                e1.record(side)
+                e1.synchronize()
+                times.append(e0.elapsed_time(e1))
+        return float(np.median(times))
+    finally:
+        side.synchronize()
+        plan.close()
+
+
+def tune_batched(count: int = 256, out: Optional[str] = None, replays: int = 10, device: str = "cuda:0"):
+    """Exhaustive search over every compiled tile x split policy; appends one
+    row per candidate to the batched database and returns the best."""
+    from . import routines as R
+    out = out or os.path.join(TUNING_DIR, "batched.csv")
+    dims = R.batched_problem_dims(count)
+    tensors = None
+    best = None
+    new = not os.path.exists(out)
+    with open(out, "a", newline="") as f:
+        w = csv.writer(f)
+        if new:
+            w.writerow(BATCHED_COLUMNS)
+        for cfg in R.hgemm_configs():
+            for policy in BATCHED_POLICIES:
+                probs, tensors = batched_problems(dims, device, policy, cfg, tensors)
+                ms = time_batched(probs, cfg, device, replays)
+                w.writerow([count, cfg.bm, cfg.bn, cfg.bk, cfg.stages, policy, f"{ms:.6f}"])
+                f.flush()
+                print(f"[batched] {cfg!r} policy {policy}: {ms:.3f} ms", flush=True)
+                if best is None or ms < best[0]:
+                    best = (ms, cfg, policy)
+    return best
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m libthom_amd.tuner")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -369,12 +478,20 @@ def main(argv=None):
     tr.add_argument("--db", default=os.path.join(TUNING_DIR, "w4a16_sweep.csv"))
     tr.add_argument("--out", default=os.path.join(TUNING_DIR, "w4a16_mlp.json"))
     tr.add_argument("--steps", type=int, default=4000)
+    bt = sub.add_parser("batched")
+    bt.add_argument("--count", type=int, default=256)
+    bt.add_argument("--out", default=os.path.join(TUNING_DIR, "batched.csv"))
+    bt.add_argument("--replays", type=int, default=10)
     ev = sub.add_parser("eval")
     ev.add_argument("--db", default=os.path.join(TUNING_DIR, "w4a16_sweep.csv"))
     ev.add_argument("--model", default=os.path.join(TUNING_DIR, "w4a16_mlp.json"))
     ev.add_argument("--default", default=os.path.join(TUNING_DIR, "w4a16_default.csv"))
     a = ap.parse_args(argv)
 
+    if a.cmd == "batched":
+        ms, cfg, policy = tune_batched(a.count, a.out, a.replays)
+        print(f"[batched] best {cfg!r} policy {policy}: {ms:.3f} ms", flush=True)
+        return 0
     if a.cmd == "sweep":
         sweeper = Sweeper()
         done = set(read_db(a.out).keys()) if os.path.exists(a.out) else set()

@@ -26,17 +26,19 @@ def make_problems(dims, dev, seed, splits=None):
 
 
 def single(A, B, cfg, split):
-    c = HgemmConfig(cfg.bm, cfg.bn, cfg.bk, split, 2)
+    c = HgemmConfig(cfg.bm, cfg.bn, cfg.bk, split, cfg.stages)
     return R.hgemm(A, B, config=c)
 
 
-@pytest.mark.parametrize("tile", [(128, 128, 64), (64, 64, 64), (256, 256, 64)])
+@pytest.mark.parametrize("tile", [(128, 128, 64, 2), (64, 64, 64, 2), (256, 256, 64, 2), (128, 128, 32, 4),
+                                  (256, 256, 32, 4), (256, 128, 64, 3)])
 def test_grouped_matches_single(dev, orc, tile):
     dims = [(64, 64, 64), (100, 200, 72), (1000, 48, 512), (48, 1000, 4096), (512, 512, 512), (16, 16, 16),
             (257, 136, 1000), (1024, 1024, 256), (80, 96, 2048), (0, 64, 64)]
     splits = [1, 1, 2, 8, 1, 1, 3, 1, 4, 1]
     probs = make_problems(dims, dev, 1, splits)
-    cfg = HgemmConfig(*tile, 1, 2)
+    bm, bn, bk, st = tile
+    cfg = HgemmConfig(bm, bn, bk, 1, st)
     plan = R.GroupedHgemm(probs, cfg)
     plan.run()
     torch.cuda.synchronize()
@@ -54,10 +56,11 @@ def test_grouped_matches_single(dev, orc, tile):
     plan.close()
 
 
-def test_grouped_graph_replays(dev):
+@pytest.mark.parametrize("stages", [2, 4])
+def test_grouped_graph_replays(dev, stages):
     dims = [(128, 256, 512), (64, 64, 4096), (300, 104, 200), (512, 64, 1024)]
     probs = make_problems(dims, dev, 2, [1, 4, 1, 2])
-    plan = R.GroupedHgemm(probs, HgemmConfig(128, 128, 64, 1, 2))
+    plan = R.GroupedHgemm(probs, HgemmConfig(128, 128, 64 if stages == 2 else 32, 1, stages))
     side = torch.cuda.Stream(device=dev)
     plan.run(side)
     side.synchronize()
@@ -79,12 +82,14 @@ def test_grouped_graph_replays(dev):
 def test_batched_configuration(dev):
     dims = R.batched_problem_dims()
     assert len(dims) == 256 and all(64 <= d <= 4096 and d % 16 == 0 for p in dims for d in p)
-    probs = make_problems(dims, dev, 3)
-    plan = R.GroupedHgemm(probs, HgemmConfig(128, 128, 64, 1, 2))
+    from libthom_amd import tuner
+    cfg, policy = tuner.best_batched(len(dims))
+    probs, _ = tuner.batched_problems(dims, dev, policy, cfg)
+    plan = R.GroupedHgemm(probs, cfg)
     plan.run()
     torch.cuda.synchronize()
     rng = np.random.default_rng(0)
     for i in rng.choice(256, 12, replace=False):
-        A, B, C, _ = probs[i]
-        assert torch.equal(C, single(A, B, HgemmConfig(128, 128, 64, 1, 2), 1)), dims[i]
+        A, B, C, split = probs[i]
+        assert torch.equal(C, single(A, B, cfg, split)), dims[i]
     plan.close()
