@@ -177,9 +177,10 @@ int lthip_hgemm_grouped_plan_destroy(lthip_grouped_plan *plan);
  *      rounding); C = fp16(sum_k A[m][k] W[k][n]) with FP32 accumulation.
  *      K % group == 0, group % 8 == 0, N % 8 == 0. ------------------------- */
 typedef struct lthip_w4a16_config {
-    int kind;        /* 0 = auto, 1 = small-M split-K stream, 2 = tiled MFMA */
-    int tile_n;      /* kind 1: 64 columns per block (fixed); kind 2: BN */
-    int tile_m;      /* kind 2: BM */
+    int kind;        /* 0 = auto, 1 = small-M split-K stream, 2 = tiled MFMA, 3 = pipelined tiled MFMA
+                        (DMA ring; group % 64 == 0, N % 32 == 0, 16-byte aligned scales/zero-points) */
+    int tile_n;      /* kind 1: 64 columns per block (fixed); kinds 2, 3: BN */
+    int tile_m;      /* kinds 2, 3: BM */
     int split_k;     /* K splits (>= 1); > 1 needs the workspace */
 } lthip_w4a16_config;
 

@@ -216,6 +216,45 @@ struct TileOut {
     unsigned* counter;   // split-K: this tile's arrival counter
 };
 
+// Read-back by the last split of a tile: for the 16 accumulator rows rows[r]
+// of one MFMA tile column col, C = alpha * sum_sp partial[sp] (+ beta * C),
+// the splits summed in order.  Loads are issued two splits (32 values) at a
+// time from clamped addresses, so no load waits for the previous one.
+__device__ __forceinline__ void collect_store16(const TileOut& out, int M, int N, int splits, const int* rows,
+                                                int col) {
+    const int cc = col < N ? col : N - 1;
+    long off[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) off[r] = (long)(rows[r] < M ? rows[r] : M - 1) * N + cc;
+    float sum[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) sum[r] = 0.0f;
+    const long slice = (long)M * N;
+    for (int sp = 0; sp < splits; sp += 2) {
+        const int sp1 = sp + 1 < splits ? sp + 1 : sp;
+        float v0[16], v1[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            v0[r] = collect(out.partial + sp * slice + off[r]);
+            v1[r] = collect(out.partial + sp1 * slice + off[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            sum[r] += v0[r];
+            if (sp + 1 < splits) sum[r] += v1[r];
+        }
+    }
+    if (col >= N) return;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        if (rows[r] >= M) continue;
+        __half* cp = out.C + (long)rows[r] * out.ldc + col;
+        float v = out.alpha * sum[r];
+        if (out.beta != 0.0f) v = fmaf(out.beta, __half2float(*cp), v);
+        *cp = __float2half_rn(v);
+    }
+}
+
 // Computes C[m0:m0+BM, n0:n0+BN] over K-slabs [s0, s1) and writes it (or its
 // split-K partial).  `smem` holds 2*(BM+BN)*BK halves.
 template <int BM, int BN, int BK, class BP>
@@ -323,20 +362,12 @@ __device__ __forceinline__ void gemm_tile(__half* smem, const __half* __restrict
 #pragma unroll
     for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < TN; j++)
+        for (int j = 0; j < TN; j++) {
+            int rows[16];
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int row = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
-                const int col = n0 + wn + 32 * j + fcol;
-                if (row < M && col < N) {
-                    float sum = 0.0f;
-                    for (int sp = 0; sp < splits; sp++) sum += collect(out.partial + ((long)sp * M + row) * N + col);
-                    __half* cp = out.C + (long)row * out.ldc + col;
-                    float v = out.alpha * sum;
-                    if (out.beta != 0.0f) v = fmaf(out.beta, __half2float(*cp), v);
-                    *cp = __float2half_rn(v);
-                }
-            }
+            for (int r = 0; r < 16; r++) rows[r] = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
+            collect_store16(out, M, N, splits, rows, n0 + wn + 32 * j + fcol);
+        }
     if (threadIdx.x == 0) reset_counter(out.counter);
 }
 

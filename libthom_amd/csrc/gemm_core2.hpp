@@ -268,8 +268,7 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
         // last k-step waits for slab s+1, with slabs s+2 .. s+STAGES-1 (all issued
         // by then) still in flight.  Past the end the last slab is fetched again
         // into a stage nobody reads, so the vmcnt arithmetic never changes.
-        constexpr int CHUNK = C::LOADS / KSTEPS;
-        static_assert(C::LOADS % KSTEPS == 0, "DMA chunks per k-step");
+        // DMA chunk kk = instructions [kk * LOADS / KSTEPS, (kk + 1) * LOADS / KSTEPS)
 #pragma unroll
         for (int p = 0; p < STAGES - 1; p++) src.template issue<0, C::LOADS>(smem, p, p < nslabs ? p : nslabs - 1);
         ring_barrier<C::LOADS * (STAGES - 2)>();
@@ -288,14 +287,14 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
             static_for<0, KSTEPS - 1>([&](auto kkc) {
                 constexpr int kk = decltype(kkc)::value;
                 fr[(kk + 1) & 1].load(sa, sb, kk + 1, wm, wn);
-                src.template issue<kk * CHUNK, (kk + 1) * CHUNK>(smem, fill_stage, fill);
+                src.template issue<kk * C::LOADS / KSTEPS, (kk + 1) * C::LOADS / KSTEPS>(smem, fill_stage, fill);
                 const Core2Frags<C>& f = fr[kk & 1];
 #pragma unroll
                 for (int i = 0; i < TM; i++)
 #pragma unroll
                     for (int j = 0; j < TN; j++)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
-                interleave<MFMAS, READS, CHUNK>();
+                interleave<MFMAS, READS, (kk + 1) * C::LOADS / KSTEPS - kk * C::LOADS / KSTEPS>();
                 __builtin_amdgcn_sched_barrier(0);
             });
             // last k-step: half the MFMAs with the last DMA chunk, hand-over,
@@ -303,13 +302,13 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
             {
                 constexpr int KL = KSTEPS - 1;
                 const Core2Frags<C>& f = fr[KL & 1];
-                src.template issue<KL * CHUNK, C::LOADS>(smem, fill_stage, fill);
+                src.template issue<KL * C::LOADS / KSTEPS, C::LOADS>(smem, fill_stage, fill);
 #pragma unroll
                 for (int i = 0; i < TM / 2; i++)
 #pragma unroll
                     for (int j = 0; j < TN; j++)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
-                interleave<MFMAS / 2, 0, CHUNK>();
+                interleave<MFMAS / 2, 0, C::LOADS - KL * C::LOADS / KSTEPS>();
                 __builtin_amdgcn_sched_barrier(0);
                 ring_barrier<C::LOADS * (STAGES - 2)>();
                 stage = stage + 1 == STAGES ? 0 : stage + 1;
@@ -391,20 +390,12 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
 #pragma unroll
     for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < TN; j++)
+        for (int j = 0; j < TN; j++) {
+            int rows[16];
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int row = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
-                const int col = n0 + wn + 32 * j + fcol;
-                if (row < M && col < N) {
-                    float sum = 0.0f;
-                    for (int sp = 0; sp < splits; sp++) sum += collect(out.partial + ((long)sp * M + row) * N + col);
-                    __half* cp = out.C + (long)row * out.ldc + col;
-                    float v = out.alpha * sum;
-                    if (out.beta != 0.0f) v = fmaf(out.beta, __half2float(*cp), v);
-                    *cp = __float2half_rn(v);
-                }
-            }
+            for (int r = 0; r < 16; r++) rows[r] = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
+            collect_store16(out, M, N, splits, rows, n0 + wn + 32 * j + fcol);
+        }
     if (threadIdx.x == 0) reset_counter(out.counter);
 }
 

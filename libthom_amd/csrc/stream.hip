@@ -133,8 +133,19 @@ __global__ __launch_bounds__(kRBlock) void sum_kernel(const float* __restrict__ 
     }
     __syncthreads();
     if (!is_last) return;
+    // this thread's block partials b = tid + kRBlock q, q ascending, all loads
+    // in flight at once (kRMaxBlocks / kRBlock of them at most)
+    constexpr int Q = (kRMaxBlocks + kRBlock - 1) / kRBlock;
+    float v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const unsigned b = threadIdx.x + kRBlock * q;
+        v[q] = collect(&partials[b < gridDim.x ? b : 0]);
+    }
     float p = 0.0f;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += kRBlock) p += collect(&partials[b]);
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        if (threadIdx.x + kRBlock * q < gridDim.x) p += v[q];
     p = block_sum<kRBlock>(p, red);
     if (threadIdx.x == 0) {
         *out = p;

@@ -1,0 +1,305 @@
+// Pipelined w4a16 tile: C (BM x BN) = A (row-major fp16) * dequant(W int4).
+//
+// The same DMA ring as gemm_core2.hpp, with the weight side replaced:
+//   A tile   [BM][64] fp16, swizzled as in gemm_core2.hpp;
+//   W tile   [8][BN] packed uint32 (8 consecutive k of one column each), as in
+//            memory, 16-byte chunk c of row r at c ^ 8(r & 1) so that the two
+//            rows one wave reads together fall in different LDS banks;
+//   SZ block, 1 KiB per wave column group: the BN/WGN scales of the slab's
+//            group (fp16) followed by their packed zero-points.
+// Every wave issues one DMA instruction per slab for its SZ block (a DMA
+// instruction writes 16 bytes for each of the 64 lanes, hence 1 KiB; waves of
+// the same columns write identical bytes), so all waves count the same number
+// of DMA instructions and the vmcnt arithmetic of the ring is unchanged.
+//
+// One uint32 of W is one MFMA B fragment: lane l of n-tile j reads row
+// 2 kk + l/32, column wn + 32 j + l%32, and dequantises it in registers to
+// fp16((q - z) * s) in k order [0,4,1,5,2,6,3,7] (dequant8, bit-exact); the A
+// fragment is permuted the same way, which leaves every dot product unchanged.
+//
+// Requirements (checked by the caller): group % 64 == 0, K % group == 0,
+// N % 32 == 0, M >= 1; A, W, scales and qzeros 16-byte aligned.
+#pragma once
+
+#include "gemm_core2.hpp"
+
+namespace lt {
+
+__device__ __forceinline__ int swzW2(int r) { return 8 * (r & 1); }
+
+template <int BM_, int BN_, int STAGES_, int WGM_, int WGN_>
+struct Core2W4 {
+    static constexpr int BM = BM_, BN = BN_, BK = 64, STAGES = STAGES_, WGM = WGM_, WGN = WGN_;
+    static constexpr int WAVES = WGM * WGN, THREADS = 64 * WAVES;
+    static constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 32, TN = WN / 32;
+    static constexpr int A_BYTES = BM * BK * 2, B_BYTES = (BK / 8) * BN * 4;
+    static constexpr int SZ_SCALE = 2 * WN, SZ_ZERO = WN / 2, SZ_STRIDE = 1024;
+    static constexpr int SZ_BYTES = WGN * SZ_STRIDE;
+    static constexpr int STAGE_BYTES = A_BYTES + B_BYTES + SZ_BYTES;
+    static constexpr int A_INSTR = A_BYTES / (1024 * WAVES), B_INSTR = B_BYTES / (1024 * WAVES);
+    static constexpr int LOADS = A_INSTR + B_INSTR + 1;
+    static constexpr int KC = BK / 8;
+    static constexpr int A_ROWS_PER = 1024 / (BK * 2);
+    static constexpr int NC = BN * 4 / 16;                  // 16-byte chunks per packed row
+    static constexpr int B_ROWS_PER = 1024 / (BN * 4);
+    static constexpr int SZ_SLANES = SZ_SCALE / 16, SZ_ZLANES = SZ_ZERO / 16;
+    static constexpr int SMEM = STAGES * STAGE_BYTES;
+    static_assert(TM >= 2 && TM % 2 == 0 && TN >= 1, "wave tile");
+    static_assert(A_BYTES % (1024 * WAVES) == 0 && B_BYTES % (1024 * WAVES) == 0, "DMA split");
+    static_assert(BN >= 64 && BN <= 256 && B_ROWS_PER >= 1, "BN");
+    static_assert(SZ_ZERO % 16 == 0 && SZ_SLANES + SZ_ZLANES <= 64, "SZ block");
+    static_assert(STAGES >= 3 && SMEM <= 160 * 1024, "LDS budget");
+    static_assert(LOADS * (STAGES - 1) < 64, "vmcnt range");
+};
+
+template <class C>
+struct W4Src {
+    const __half* a[C::A_INSTR];
+    const uint32_t* w[C::B_INSTR];
+    const char* sz;          // this lane's SZ source in group 0 (scales or zero-points)
+    long sz_group_step;      // bytes between consecutive groups for that source
+    long w_step;             // uint32 between consecutive slabs of W
+    int group_slabs;         // slabs per quantisation group
+    int slab0;               // absolute index of slab 0 of this range
+
+    __device__ __forceinline__ void init(const __half* __restrict__ A, long lda, const BW4Params& p, int M, int N,
+                                         int m0, int n0, int k0) {
+        const int lane = threadIdx.x & 63;
+        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+#pragma unroll
+        for (int i = 0; i < C::A_INSTR; i++) {
+            const int inst = wave * C::A_INSTR + i;
+            const int row = inst * C::A_ROWS_PER + lane / C::KC;
+            const int kc = (lane % C::KC) ^ swzA2<C::BK>(row);
+            int gm = m0 + row;
+            gm = gm < M ? gm : M - 1;
+            a[i] = A + (long)gm * lda + k0 + kc * 8;
+        }
+#pragma unroll
+        for (int i = 0; i < C::B_INSTR; i++) {
+            const int inst = wave * C::B_INSTR + i;
+            const int row = inst * C::B_ROWS_PER + lane / C::NC;
+            const int ch = (lane % C::NC) ^ swzW2(row);
+            int gn = n0 + ch * 4;
+            gn = gn < N ? gn : N - 4;
+            w[i] = p.W + (long)(k0 / 8 + row) * N + gn;
+        }
+        w_step = (long)(C::BK / 8) * N;
+        group_slabs = p.group / C::BK;
+        slab0 = k0 / C::BK;
+        // SZ: lanes [0, SZ_SLANES) fetch 8 scales each, the next SZ_ZLANES 4
+        // packed zero-point words each (32 columns); the rest repeat lane 0
+        const int wn = (wave % C::WGN) * C::WN;
+        const int sl = lane < C::SZ_SLANES + C::SZ_ZLANES ? lane : 0;
+        if (sl < C::SZ_SLANES || !p.qzeros) {
+            int gn = n0 + wn + 8 * (sl < C::SZ_SLANES ? sl : 0);
+            gn = gn < N ? gn : N - 8;
+            sz = reinterpret_cast<const char*>(p.scales + gn);
+            sz_group_step = (long)N * 2;
+        } else {
+            int zw = (n0 + wn) / 8 + 4 * (sl - C::SZ_SLANES);
+            zw = zw < N / 8 ? zw : N / 8 - 4;
+            sz = reinterpret_cast<const char*>(p.qzeros + zw);
+            sz_group_step = (long)(N / 8) * 4;
+        }
+    }
+
+    template <int Q0, int Q1>
+    __device__ __forceinline__ void issue(char* smem, int stage, int slab) const {
+        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        char* sa = smem + stage * C::STAGE_BYTES;
+        char* sb = sa + C::A_BYTES;
+        char* ss = sb + C::B_BYTES;
+#pragma unroll
+        for (int q = Q0; q < Q1; q++) {
+            if (q < C::A_INSTR) {
+                __builtin_amdgcn_global_load_lds(a[q] + (long)slab * C::BK,
+                                                 (lds_void_t*)(lds_char_t*)(sa + (wave * C::A_INSTR + q) * 1024), 16,
+                                                 0, 0);
+            } else if (q < C::A_INSTR + C::B_INSTR) {
+                const int i = q - C::A_INSTR;
+                __builtin_amdgcn_global_load_lds(w[i] + (long)slab * w_step,
+                                                 (lds_void_t*)(lds_char_t*)(sb + (wave * C::B_INSTR + i) * 1024), 16,
+                                                 0, 0);
+            } else {
+                const int cg = wave % C::WGN;
+                __builtin_amdgcn_global_load_lds(sz + (long)((slab0 + slab) / group_slabs) * sz_group_step,
+                                                 (lds_void_t*)(lds_char_t*)(ss + cg * C::SZ_STRIDE), 16, 0, 0);
+            }
+        }
+    }
+};
+
+template <class C>
+struct W4Frags {
+    v8h a[C::TM], b[C::TN];
+
+    // zz[j] = fp16(1024 + z) in both halves, s2[j] = scale in both halves
+    __device__ __forceinline__ void load(const char* sa, const char* sb, int kk, int wm, int wn, const uint32_t* zz,
+                                         const uint32_t* s2) {
+        const int lane = threadIdx.x & 63;
+        const int frow = lane & 31, fkc = lane >> 5;
+#pragma unroll
+        for (int i = 0; i < C::TM; i++) {
+            const int row = wm + 32 * i + frow;
+            const int kc = (2 * kk + fkc) ^ swzA2<C::BK>(row);
+            const uint4 raw = *reinterpret_cast<const uint4*>(sa + row * (C::BK * 2) + kc * 16);
+            const uint4 pr = perm_k8(raw);
+            __builtin_memcpy(&a[i], &pr, 16);
+        }
+        const int r = 2 * kk + fkc;
+#pragma unroll
+        for (int j = 0; j < C::TN; j++) {
+            const int c = wn + 32 * j + frow;
+            const uint32_t wv =
+                *reinterpret_cast<const uint32_t*>(sb + r * (C::BN * 4) + (((c >> 2) ^ swzW2(r)) << 4) + (c & 3) * 4);
+            const uint4 d = dequant8(wv, zz[j], s2[j]);
+            __builtin_memcpy(&b[j], &d, 16);
+        }
+    }
+};
+
+// scales and zero-points of this lane's columns for the slab in stage `ss`
+template <class C>
+__device__ __forceinline__ void w4_read_sz(const char* ss, int cg, bool sym, uint32_t* zz, uint32_t* s2) {
+    const int lane = threadIdx.x & 63;
+    const char* blk = ss + cg * C::SZ_STRIDE;
+#pragma unroll
+    for (int j = 0; j < C::TN; j++) {
+        const int c = 32 * j + (lane & 31);   // column within the wave's WN
+        const uint32_t s = *reinterpret_cast<const uint16_t*>(blk + 2 * c);
+        uint32_t z = 8;
+        if (!sym) z = (*reinterpret_cast<const uint32_t*>(blk + C::SZ_SCALE + 4 * (c >> 3)) >> (4 * (c & 7))) & 0xFu;
+        zz[j] = (0x6400u + z) * 0x10001u;
+        s2[j] = s | (s << 16);
+    }
+}
+
+// C tile at (m0, n0) over K-slabs [s0, s1) of 64 (K % 64 == 0); split-K
+// protocol as gemm_tile2 (alpha 1, beta 0).
+template <class C>
+__device__ __forceinline__ void w4_tile2(char* smem, const __half* __restrict__ A, long lda, const BW4Params& p, int M,
+                                         int N, int m0, int n0, int s0, int s1, int split, int splits,
+                                         const TileOut& out) {
+    constexpr int TM = C::TM, TN = C::TN, BK = C::BK, STAGES = C::STAGES, KSTEPS = BK / 16;
+    constexpr int MFMAS = TM * TN, READS = TM + TN;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int cg = wave % C::WGN;
+    const int wm = (wave / C::WGN) * C::WM, wn = cg * C::WN;
+    const bool sym = p.qzeros == nullptr;
+
+    v16f acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
+
+    const int nslabs = s1 - s0;
+    if (nslabs > 0) {
+        W4Src<C> src;
+        src.init(A, lda, p, M, N, m0, n0, s0 * BK);
+        wait_vmcnt<0>();
+#pragma unroll
+        for (int q = 0; q < STAGES - 1; q++) src.template issue<0, C::LOADS>(smem, q, q < nslabs ? q : nslabs - 1);
+        ring_barrier<C::LOADS * (STAGES - 2)>();
+
+        uint32_t zz[TN], s2[TN];
+        W4Frags<C> fr[2];
+        w4_read_sz<C>(smem + C::A_BYTES + C::B_BYTES, cg, sym, zz, s2);
+        fr[0].load(smem, smem + C::A_BYTES, 0, wm, wn, zz, s2);
+        int stage = 0, fill_stage = STAGES - 1;
+        for (int s = 0; s < nslabs; s++) {
+            const char* sa = smem + stage * C::STAGE_BYTES;
+            const char* sb = sa + C::A_BYTES;
+            const int fill = s + STAGES - 1 < nslabs ? s + STAGES - 1 : nslabs - 1;
+            __builtin_amdgcn_sched_barrier(0);
+            static_for<0, KSTEPS - 1>([&](auto kkc) {
+                constexpr int kk = decltype(kkc)::value;
+                fr[(kk + 1) & 1].load(sa, sb, kk + 1, wm, wn, zz, s2);
+                src.template issue<kk * C::LOADS / KSTEPS, (kk + 1) * C::LOADS / KSTEPS>(smem, fill_stage, fill);
+                const W4Frags<C>& f = fr[kk & 1];
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+                interleave<MFMAS, READS, (kk + 1) * C::LOADS / KSTEPS - kk * C::LOADS / KSTEPS>();
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            {
+                constexpr int KL = KSTEPS - 1;
+                const W4Frags<C>& f = fr[KL & 1];
+                src.template issue<KL * C::LOADS / KSTEPS, C::LOADS>(smem, fill_stage, fill);
+#pragma unroll
+                for (int i = 0; i < TM / 2; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+                interleave<MFMAS / 2, 0, C::LOADS - KL * C::LOADS / KSTEPS>();
+                __builtin_amdgcn_sched_barrier(0);
+                ring_barrier<C::LOADS * (STAGES - 2)>();
+                stage = stage + 1 == STAGES ? 0 : stage + 1;
+                fill_stage = fill_stage + 1 == STAGES ? 0 : fill_stage + 1;
+                const char* na = smem + stage * C::STAGE_BYTES;
+                w4_read_sz<C>(na + C::A_BYTES + C::B_BYTES, cg, sym, zz, s2);
+                fr[0].load(na, na + C::A_BYTES, 0, wm, wn, zz, s2);   // KSTEPS is even
+#pragma unroll
+                for (int i = TM / 2; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+                interleave<MFMAS / 2, READS, 0>();
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        wait_vmcnt<0>();
+    }
+    __syncthreads();
+
+    const int fcol = lane & 31, frow4 = 4 * (lane >> 5);
+    if (splits <= 1) {
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int row = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
+                    const int col = n0 + wn + 32 * j + fcol;
+                    if (row < M && col < N) out.C[(long)row * out.ldc + col] = __float2half_rn(acc[i][j][r]);
+                }
+        return;
+    }
+    float* mine = out.partial + (long)split * M * N;
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int row = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
+                const int col = n0 + wn + 32 * j + fcol;
+                if (row < M && col < N) publish(mine + (long)row * N + col, acc[i][j][r]);
+            }
+    int* last_split = reinterpret_cast<int*>(smem);
+    drain();
+    __syncthreads();
+    if (threadIdx.x == 0) *last_split = (take_ticket(out.counter) == (unsigned)splits - 1);
+    __syncthreads();
+    if (!*last_split) return;
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+            int rows[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) rows[r] = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
+            collect_store16(out, M, N, splits, rows, n0 + wn + 32 * j + fcol);
+        }
+    if (threadIdx.x == 0) reset_counter(out.counter);
+}
+
+}  // namespace lt

@@ -17,7 +17,12 @@
 //
 // kind 2, large M: the shared tiled core (gemm_core.hpp) with the int4 loader,
 // which dequantises each K-slab of W into LDS once per block.
-#include "gemm_core.hpp"
+//
+// kind 3, large M: the pipelined tile of w4_core2.hpp (A, packed W, scales and
+// zero-points reach LDS by DMA; W is dequantised in registers, one uint32 per
+// MFMA fragment).  Needs group % 64 == 0, N % 32 == 0 and 16-byte aligned
+// scales/zero-points; the automatic choice falls back to kind 2 otherwise.
+#include "w4_core2.hpp"
 
 namespace lt {
 namespace {
@@ -184,11 +189,23 @@ __global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restr
     for (int e = threadIdx.x; e < rows * (kStreamCols / 4); e += 256) {
         const int m = e / (kStreamCols / 4), n = n0 + 4 * (e % (kStreamCols / 4));
         if (n >= N) continue;
+        // splits summed in order, four splits' loads in flight at a time
         float v[4] = {0.f, 0.f, 0.f, 0.f};
-        for (int sp = 0; sp < splits; sp++) {
-            const float* src = partial + ((long)sp * M + m) * N + n;
+        for (int sp = 0; sp < splits; sp += 4) {
+            float x[4][4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) v[j] += collect(src + j);
+            for (int u = 0; u < 4; u++) {
+                const int q = sp + u < splits ? sp + u : splits - 1;
+                const float* src = partial + ((long)q * M + m) * N + n;
+#pragma unroll
+                for (int j = 0; j < 4; j++) x[u][j] = collect(src + j);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (sp + u < splits) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) v[j] += x[u][j];
+                }
         }
         const uint32_t lo = (uint32_t)__half_as_ushort(__float2half_rn(v[0])) |
                             ((uint32_t)__half_as_ushort(__float2half_rn(v[1])) << 16);
@@ -237,19 +254,102 @@ const TiledEntry kTiled[] = {
 };
 constexpr int kNumTiled = sizeof(kTiled) / sizeof(kTiled[0]);
 
+// Blocks [0, full) run whole tiles (split blockIdx.y of gridDim.y); with a
+// tail split, block full + j runs split j % tail of tile full + j / tail, so
+// the tiles left over after the last full round of launch slots are shared
+// by all slots instead of leaving most of them idle.
+template <class C>
+__global__ __launch_bounds__(C::THREADS) void w4a16_tiled2_kernel(const __half* __restrict__ A, BW4Params p,
+                                                                   __half* __restrict__ Cm, int M, int N, int K,
+                                                                   int tiles_m, int full, int tail,
+                                                                   float* __restrict__ partial,
+                                                                   unsigned* __restrict__ counters) {
+    __shared__ __align__(16) char smem[C::SMEM];
+    int t, split, splits;
+    if ((int)blockIdx.x < full) {
+        t = xcd_remap(blockIdx.x, full);
+        split = blockIdx.y;
+        splits = gridDim.y;
+    } else {
+        const int j = blockIdx.x - full;
+        t = full + j / tail;
+        split = j % tail;
+        splits = tail;
+    }
+    const int tm = t % tiles_m, tn = t / tiles_m;
+    TileOut o;
+    o.C = Cm;
+    o.ldc = N;
+    o.alpha = 1.0f;
+    o.beta = 0.0f;
+    o.partial = partial;
+    o.counter = counters ? counters + t : nullptr;
+    const int slabs = K / C::BK;
+    const int s0 = (int)((long)slabs * split / splits), s1 = (int)((long)slabs * (split + 1) / splits);
+    w4_tile2<C>(smem, A, (long)K, p, M, N, tm * C::BM, tn * C::BN, s0, s1, split, splits, o);
+}
+
+typedef void (*tiled2_fn)(const __half*, BW4Params, __half*, int, int, int, int, int, int, float*, unsigned*);
+struct Tiled2Entry {
+    int bm, bn, smem;
+    tiled2_fn fn;
+    int threads;
+};
+#define LT_W4T2(bm, bn, st, wm, wn)                                                                    \
+    {bm, bn, Core2W4<bm, bn, st, wm, wn>::SMEM, w4a16_tiled2_kernel<Core2W4<bm, bn, st, wm, wn>>, \
+     Core2W4<bm, bn, st, wm, wn>::THREADS},
+const Tiled2Entry kTiled2[] = {LT_W4T2(256, 256, 3, 2, 2) LT_W4T2(256, 128, 3, 2, 2) LT_W4T2(128, 256, 3, 2, 2)
+                                   LT_W4T2(128, 128, 3, 2, 2)};
+#undef LT_W4T2
+constexpr int kNumTiled2 = sizeof(kTiled2) / sizeof(kTiled2[0]);
+
+const Tiled2Entry* find_tiled2(int bm, int bn) {
+    for (int i = 0; i < kNumTiled2; i++)
+        if (kTiled2[i].bm == bm && kTiled2[i].bn == bn) return &kTiled2[i];
+    return nullptr;
+}
+
+// Tail split of a kind-3 launch with split_k == 1: `tail` splits for the
+// tiles after the last full round of launch slots (1: none).
+int tiled2_tail(const Tiled2Entry& e, long M, long N, long K, int* full) {
+    const long tiles = ((M + e.bm - 1) / e.bm) * ((N + e.bn - 1) / e.bn);
+    const long per_cu = 163840 / (e.smem + 64);
+    const long slots = (long)cu_count() * (per_cu >= 1 ? per_cu : 1);
+    const long rem = tiles % slots;
+    *full = (int)tiles;
+    if (rem == 0 || tiles > kMaxCounters) return 1;
+    long tail = slots / rem;
+    if (tail > 8) tail = 8;
+    if (tail > K / 64 / 4) tail = K / 64 / 4;
+    if (tail < 2) return 1;
+    *full = (int)(tiles - rem);
+    return (int)tail;
+}
+
 void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_w4a16_config* out) {
     lthip_w4a16_config c = cfg ? *cfg : lthip_w4a16_config{0, 0, 0, 0};
     if (c.kind == 0) {
         if (M <= 64) {
+            // split K (FP32 partials) only for the thinnest problems: for a
+            // few rows the weight stream needs more blocks than the column
+            // tiles give, from 16 rows on the partial traffic costs more
             c.kind = 1;
             const long ntiles = (long)((N + kStreamCols - 1) / kStreamCols);
             const long ksteps = (long)((K / 8 + 3) / 4);
-            long split = (4L * cu_count() + ntiles - 1) / ntiles;
+            long split = M <= 8 ? (2L * cu_count() + ntiles - 1) / ntiles : 1;
+            if (split > 4) split = 4;
             if (split > ksteps / 4) split = ksteps / 4;
             if (split < 1) split = 1;
             c.split_k = (int)split;
             c.tile_n = kStreamCols;
             c.tile_m = 16;
+        } else if (N % 32 == 0) {
+            // pipelined tiles: the largest tile once M fills it; the launch
+            // shares the last partial round of tiles out by splitting K
+            c.kind = 3;
+            c.tile_m = M > 128 ? (M > 512 ? 256 : 128) : 128;
+            c.tile_n = M > 128 ? 256 : 128;
+            c.split_k = 1;
         } else {
             c.kind = 2;
             c.tile_m = M >= 1024 ? 256 : (M > 128 ? 128 : 64);
@@ -274,8 +374,14 @@ extern "C" {
 size_t lthip_w4a16_workspace_size(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg) {
     lthip_w4a16_config c;
     lt::resolve(M, N, K, cfg, &c);
-    if (c.split_k <= 1) return 0;
-    return lt::kCounterBytes + sizeof(float) * (size_t)c.split_k * M * N;
+    int splits = c.split_k;
+    if (c.kind == 3 && splits <= 1) {
+        const lt::Tiled2Entry* e = lt::find_tiled2(c.tile_m, c.tile_n);
+        int full = 0;
+        if (e) splits = lt::tiled2_tail(*e, (long)M, (long)N, (long)K, &full);
+    }
+    if (splits <= 1) return 0;
+    return lt::kCounterBytes + sizeof(float) * (size_t)splits * M * N;
 }
 
 int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t* scales, const uint32_t* qzeros,
@@ -332,6 +438,54 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
                                counters);
         LT_TRY_LAUNCH("w4a16 stream launch");
         return LTHIP_OK;
+    }
+    if (c.kind == 3) {
+        int idx2 = -1;
+        for (int i = 0; i < kNumTiled2; i++)
+            if (kTiled2[i].bm == c.tile_m && kTiled2[i].bn == c.tile_n) idx2 = i;
+        const bool ok = idx2 >= 0 && group % 64 == 0 && N % 32 == 0 &&
+                        reinterpret_cast<uintptr_t>(scales) % 16 == 0 &&
+                        reinterpret_cast<uintptr_t>(qzeros) % 16 == 0;
+        if (!ok && cfg && cfg->kind == 3)
+            return fail(LTHIP_ERR_UNSUPPORTED,
+                        "w4a16: kind 3 needs a built tile, group %% 64 == 0, N %% 32 == 0 and 16-byte aligned "
+                        "scales/zero-points");
+        if (ok) {
+            const Tiled2Entry& e = kTiled2[idx2];
+            const int tiles_m = (m + c.tile_m - 1) / c.tile_m, tiles_n = (n + c.tile_n - 1) / c.tile_n;
+            const int tiles = tiles_m * tiles_n;
+            int splits = c.split_k;
+            const int slabs = k / 64;
+            if (splits > slabs) splits = slabs > 0 ? slabs : 1;
+            if ((size_t)tiles > kMaxCounters) splits = 1;
+            int full = tiles, tail = 1;
+            if (splits <= 1) {
+                tail = tiled2_tail(e, m, n, k, &full);
+                // the tail split needs the workspace; without one the launch runs whole tiles
+                if (tail > 1 && (!ws || ws_bytes < kCounterBytes + sizeof(float) * (size_t)tail * M * N ||
+                                 reinterpret_cast<uintptr_t>(ws) % 16)) {
+                    tail = 1;
+                    full = tiles;
+                }
+            }
+            float* partial = nullptr;
+            unsigned* counters = nullptr;
+            if (splits > 1 || tail > 1) {
+                const size_t need = kCounterBytes + sizeof(float) * (size_t)(splits > tail ? splits : tail) * M * N;
+                if (!ws || ws_bytes < need) return fail(LTHIP_ERR_WORKSPACE, "w4a16: need %zu workspace bytes", need);
+                if (reinterpret_cast<uintptr_t>(ws) % 16)
+                    return fail(LTHIP_ERR_INVALID_ARG, "w4a16: workspace alignment");
+                counters = reinterpret_cast<unsigned*>(ws);
+                partial = reinterpret_cast<float*>(reinterpret_cast<char*>(ws) + kCounterBytes);
+            }
+            const unsigned blocks = (unsigned)(full + (tiles - full) * tail);
+            hipLaunchKernelGGL(e.fn, dim3(blocks, (unsigned)splits), dim3((unsigned)e.threads), 0, s,
+                               reinterpret_cast<const __half*>(A), p, reinterpret_cast<__half*>(C), m, n, k, tiles_m,
+                               full, tail, partial, counters);
+            LT_TRY_LAUNCH("w4a16 tiled2 launch");
+            return LTHIP_OK;
+        }
+        c.kind = 2;   // automatic choice on a shape kind 3 does not take: same tile and split on kind 2
     }
     if (c.kind != 2) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: unknown kind %d", c.kind);
     int idx = -1;

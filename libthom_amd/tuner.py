@@ -47,17 +47,26 @@ TUNING_DIR = os.path.join(HERE, "tuning")
 DB_COLUMNS = ["M", "N", "K", "group", "kind", "tile_n", "tile_m", "split_k", "ms"]
 
 # the search space: index i is output i of the cost model
+# (kind, tile_n, tile_m, split_k); kind 3 with split_k 1 lets the library
+# split K for the last partial round of tiles
 W4A16_SPACE: List[Tuple[int, int, int, int]] = (
     [(1, 64, 16, s) for s in (1, 2, 3, 4, 6, 8, 12, 16)] +
     [(2, 64, 64, s) for s in (1, 2, 4, 8)] + [(2, 128, 64, s) for s in (1, 2, 4, 8)] +
     [(2, 128, 128, s) for s in (1, 2, 4)] +
-    [(2, bn, bm, 1) for bm, bn in ((128, 256), (256, 128), (256, 256))])
+    [(2, bn, bm, 1) for bm, bn in ((128, 256), (256, 128), (256, 256))] +
+    [(3, 128, 128, 1), (3, 128, 128, 2), (3, 256, 128, 1), (3, 128, 256, 1), (3, 256, 256, 1)])
 
 
-def valid(cfg: Tuple[int, int, int, int], M: int, K: int) -> bool:
+def valid(cfg: Tuple[int, int, int, int], M: int, K: int, N: Optional[int] = None,
+          group: Optional[int] = None) -> bool:
+    """Whether the library runs cfg on the shape; kind 3 needs N % 32 == 0 and
+    group % 64 == 0 (false when N or group is not given)."""
     kind, _, _, split = cfg
     if kind == 1:
         return M <= 128 and split <= max(1, K // 32)
+    if kind == 3:
+        return N is not None and group is not None and N % 32 == 0 and group % 64 == 0 and \
+            split <= max(1, K // 256)
     return split <= max(1, K // 256)
 
 
@@ -151,7 +160,7 @@ class Sweeper:
         M, N, K, group = shape
         best, best_ms = None, math.inf
         for cfg in W4A16_SPACE:
-            if not valid(cfg, M, K):
+            if not valid(cfg, M, K, N, group):
                 continue
             ms = self.time(shape, cfg, runs)
             if db:
@@ -194,7 +203,7 @@ class CostModel:
         for r, s in enumerate(shapes):
             for c, cfg in enumerate(W4A16_SPACE):
                 ms = db[s].get(cfg)
-                if ms is not None and valid(cfg, s[0], s[2]):
+                if ms is not None and valid(cfg, s[0], s[2], s[1], s[3]):
                     Y[r, c] = math.log(ms)
                     mask[r, c] = 1
         return X, Y, mask
@@ -269,7 +278,7 @@ class CostModel:
         logt = self.predict_logtime(shapes) if logt is None else logt
         out = []
         for s, row in zip(shapes, logt):
-            cands = [(row[i], cfg) for i, cfg in enumerate(W4A16_SPACE) if valid(cfg, s[0], s[2])]
+            cands = [(row[i], cfg) for i, cfg in enumerate(W4A16_SPACE) if valid(cfg, s[0], s[2], s[1], s[3])]
             out.append(min(cands)[1])
         return out
 

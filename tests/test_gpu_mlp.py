@@ -63,7 +63,7 @@ def test_training_fits_a_known_cost_surface(dev):
     for s in shapes:
         f = tuner.features(*s) / 10.0
         db[s] = {cfg: float(np.exp(A[i] @ f + bias[i])) for i, cfg in enumerate(tuner.W4A16_SPACE)
-                 if tuner.valid(cfg, s[0], s[2])}
+                 if tuner.valid(cfg, s[0], s[2], s[1], s[3])}
     model = tuner.CostModel(device="cuda:0")
     hist = model.fit(db, shapes[:500], steps=3000, log_every=500)
     assert hist[-1] < 0.02 * hist[0]

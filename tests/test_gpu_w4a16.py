@@ -122,3 +122,43 @@ def test_w4a16_symmetric_zero_point(dev, orc):
     for cfg in (None, W4A16Config(2, 64, 64, 1)):
         C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), None, 128, config=cfg).cpu().numpy()
         w4_close(C, ref)
+
+
+KIND3 = [W4A16Config(3, 256, 256, 1), W4A16Config(3, 128, 256, 1), W4A16Config(3, 256, 128, 1),
+         W4A16Config(3, 128, 128, 1), W4A16Config(3, 128, 128, 3), W4A16Config(3, 256, 256, 2)]
+
+
+@pytest.mark.parametrize("cfg", KIND3)
+@pytest.mark.parametrize("M", [1, 65, 130, 300])
+@pytest.mark.parametrize("group", [64, 128, 256])
+def test_w4a16_pipelined(dev, orc, cfg, M, group):
+    N, K = 544, 1024
+    A, q, s, z = inputs.w4a16_problem(M, N, K, group, seed=91 + M)
+    packed, qz = inputs.pack_w4(q), inputs.pack_zeros(z)
+    C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), group, config=cfg).cpu().numpy()
+    ref = orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, group))
+    w4_close(C, ref)
+
+
+def test_w4a16_pipelined_exact_and_symmetric(dev, orc):
+    M, N, K = 200, 288, 512
+    rng = np.random.default_rng(15)
+    q = rng.integers(0, 16, (K, N)).astype(np.uint8)
+    z = rng.integers(0, 16, (K // 128, N)).astype(np.uint8)
+    s = np.ones((K // 128, N), np.float16)
+    A = rng.integers(-2, 3, (M, K)).astype(np.float16)
+    packed, qz = orc.w4_pack(q), orc.w4_pack_zeros(z)
+    for cfg in KIND3:
+        C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), 128, config=cfg).cpu().numpy()
+        assert np.array_equal(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, 128))), cfg
+        C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), None, 128, config=cfg).cpu().numpy()
+        assert np.array_equal(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, None, 128))), cfg
+
+
+def test_w4a16_pipelined_rejects_unsupported(dev):
+    from libthom_amd._lib import LthipError
+    M, N, K = 64, 520, 256
+    A, q, s, z = inputs.w4a16_problem(M, N, K, 128, seed=5)
+    with pytest.raises(LthipError):
+        R.w4a16(t(A, dev), i32(inputs.pack_w4(q), dev), t(s, dev), i32(inputs.pack_zeros(z), dev), 128,
+                config=W4A16Config(3, 128, 128, 1))

@@ -24,9 +24,14 @@ def test_db_roundtrip(tmp_path):
 
 
 def test_space_and_validity():
-    assert len(tuner.W4A16_SPACE) == 22
+    assert len(tuner.W4A16_SPACE) == 27
     assert tuner.valid((1, 64, 16, 8), 128, 4096) and not tuner.valid((1, 64, 16, 8), 129, 4096)
     assert tuner.valid((2, 256, 256, 1), 1, 512)
+    # kind 3: N % 32 == 0 and group % 64 == 0, known only with N and group
+    assert tuner.valid((3, 256, 256, 1), 2048, 4096, 11008, 128)
+    assert not tuner.valid((3, 256, 256, 1), 2048, 4096, 11008, 32)
+    assert not tuner.valid((3, 256, 256, 1), 2048, 4096, 520, 128)
+    assert not tuner.valid((3, 256, 256, 1), 2048, 4096)
     shapes = tuner.random_shapes(500)
     for M, N, K, G in shapes:
         assert 1 <= M <= 4096 and N % 64 == 0 and 512 <= N <= 16384 and K % 128 == 0 and K % G == 0
