@@ -6,7 +6,11 @@ matrix.  `value` is the achieved fraction of the 8 TB/s HBM peak, in percent;
 `roofline` carries the absolute GB/s.  The `routines` object adds the other
 routines at their configured shapes (reduction n=1e6 and 5e8, sdot, native
 dgemm 4096^3, hgemm 4096^3, w4a16 M in {1,16,128,2048}).  `cpu_baseline`
-times the oracle's sgemv on the host (one thread).
+times the oracle's sgemv on the host (one thread).  `measured_peaks` holds
+what this device delivers (MFMA probe loops, a read-only HBM stream and a
+copy); every routine carries its fraction of the datasheet peak and
+`frac_measured`, its fraction of the measured one.  `vendor_reference` times
+torch.matmul (hipBLASLt / rocBLAS) on the GEMM shapes, for context only.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -219,6 +223,47 @@ def bench_routines(torch, R, timer, dev, quick):
     return out
 
 
+def add_measured_fractions(routines, peaks):
+    """frac_measured next to each datasheet fraction: GB/s over the measured
+    HBM read rate, TFLOP/s over the measured MFMA rate of the routine's type."""
+    def walk(name, d):
+        if not isinstance(d, dict):
+            return
+        if "GB/s" in d:
+            d["frac_measured_hbm"] = round(d["GB/s"] / peaks["hbm_read_gbs"], 4)
+        if "frac_fp16_mfma" in d:
+            d["frac_measured_fp16"] = round(d["TFLOP/s"] / peaks["fp16_tflops"], 4)
+        if "frac_fp64_mfma" in d:
+            d["frac_measured_fp64"] = round(d["TFLOP/s"] / peaks["fp64_tflops"], 4)
+        if "frac_fp8_mfma" in d:
+            d["frac_measured_fp8"] = round(d["fp8_TFLOP/s"] / peaks["fp8_tflops"], 4)
+        for k, v in d.items():
+            walk(k, v)
+    for k, v in routines.items():
+        walk(k, v)
+
+
+def vendor_reference(torch, timer, dev):
+    """torch.matmul (hipBLASLt / rocBLAS) on the hgemm and dgemm shapes."""
+    out = {"note": "vendor library through torch.matmul, for context; not this library"}
+    for n in (4096, 8192):
+        a = torch.randn(n, n, device=dev, dtype=torch.float16)
+        b = torch.randn(n, n, device=dev, dtype=torch.float16)
+        torch.matmul(a, b)
+        ms = timer.per_launch_ms(lambda: torch.matmul(a, b), 10)
+        out[f"fp16_matmul_{n}"] = {"ms": round(ms, 4), "TFLOP/s": round(2 * n ** 3 / ms / 1e9, 1)}
+        del a, b
+    n = 4096
+    a = torch.randn(n, n, device=dev, dtype=torch.float64)
+    b = torch.randn(n, n, device=dev, dtype=torch.float64)
+    torch.matmul(a, b)
+    ms = timer.per_launch_ms(lambda: torch.matmul(a, b), 5)
+    out[f"fp64_matmul_{n}"] = {"ms": round(ms, 4), "TFLOP/s": round(2 * n ** 3 / ms / 1e9, 2)}
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,11 +322,17 @@ def main():
         del src, dst
 
     routines = None
+    peaks = None
     if not args.no_routines:
         del A
         torch.cuda.empty_cache()
+        log("measured peaks (MFMA probe loops, HBM read stream, copy)")
+        peaks = R.measured_peaks(dev)
+        log(f"measured peaks: {peaks}")
         log("per-routine table")
         routines = bench_routines(torch, R, timer, dev, args.quick)
+        add_measured_fractions(routines, peaks)
+        routines["vendor_reference"] = vendor_reference(torch, timer, dev)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle sgemv, 1 thread)")
@@ -300,8 +351,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("sgemv_16384x8192")},
         "cpu_baseline": cpu,
+        "measured_peaks": peaks,
         "routines": routines,
     }
+    if peaks:
+        line["roofline"]["measured_peak"] = peaks["hbm_read_gbs"]
+        line["roofline"]["frac_measured"] = round(gbs / peaks["hbm_read_gbs"], 4)
     print(json.dumps(line), flush=True)
 
 

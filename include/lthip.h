@@ -62,6 +62,19 @@ int lthip_fill_uniform_f64(double *out, size_t n, double lo, double hi, uint64_t
 /* Plain streaming copy dst[i] = src[i] (bandwidth probe and cache flush). */
 int lthip_copy_f32(float *dst, const float *src, size_t n, void *stream);
 
+/* ---- peak probes (no counterpart in the reference, which divides by the
+ *      theoretical bandwidth of common.cpp:28-34): the rates this device
+ *      delivers, for "fraction of measured peak".
+ *      lthip_probe_mfma: `blocks` blocks of 4 waves, each wave `iters` rounds
+ *      of independent MFMAs on registers; kind 0 = v_mfma_f32_32x32x16_f16,
+ *      1 = v_mfma_scale_f32_32x32x64_f8f6f4 (fp8), 2 = v_mfma_f64_16x16x4_f64.
+ *      sink holds blocks * 256 floats.  lthip_probe_mfma_flops gives the flops
+ *      of one such launch.  lthip_probe_read_f32: read-only stream over x
+ *      (16-byte aligned, n % 4 == 0); sink holds blocks * 256 floats. ------- */
+double lthip_probe_mfma_flops(int kind, int blocks, int iters);
+int lthip_probe_mfma(int kind, int blocks, int iters, float *sink, void *stream);
+int lthip_probe_read_f32(const float *x, size_t n, float *sink, int blocks, void *stream);
+
 /* ---- reduction: replaces ReductionImpl float(*)(const float*, float*, int)
  *      (reduction.cpp:20; reduction_warp.cu:83).  *out = sum x[0..n) in FP32,
  *      out is a device pointer.  Deterministic for a given n. -------------- */

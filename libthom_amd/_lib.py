@@ -60,6 +60,9 @@ _SIGS = {
     "lthip_fill_uniform_f16": (c_int, [c_vp, c_size, c_dbl, c_dbl, c_u64, c_u64, c_vp]),
     "lthip_fill_uniform_f64": (c_int, [c_vp, c_size, c_dbl, c_dbl, c_u64, c_u64, c_vp]),
     "lthip_copy_f32": (c_int, [c_vp, c_vp, c_size, c_vp]),
+    "lthip_probe_mfma_flops": (ctypes.c_double, [c_int, c_int, c_int]),
+    "lthip_probe_mfma": (c_int, [c_int, c_int, c_int, c_vp, c_vp]),
+    "lthip_probe_read_f32": (c_int, [c_vp, c_size, c_vp, c_int, c_vp]),
     "lthip_reduce_sum_f32_workspace_size": (c_size, [c_size]),
     "lthip_reduce_sum_f32": (c_int, [c_vp, c_size, c_vp, c_vp, c_size, c_vp]),
     "lthip_sdot_f32_workspace_size": (c_size, [c_size]),

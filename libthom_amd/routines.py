@@ -299,3 +299,48 @@ def batched_problem_dims(count: int = 256, seed: int = 1234):
     from . import inputs
     d = inputs.log_uniform_dims(3 * count, seed=seed)
     return [(int(d[3 * i]), int(d[3 * i + 1]), int(d[3 * i + 2])) for i in range(count)]
+
+
+def measured_peaks(device=None, read_bytes: int = 2 << 30) -> dict:
+    """What this device delivers: dense MFMA TFLOP/s (fp16, fp8, fp64; best
+    of a few launch widths, lthip_probe_mfma) and HBM read and copy GB/s
+    (lthip_probe_read_f32, lthip_copy_f32 over buffers far larger than the
+    caches)."""
+    device = torch.device(device or "cuda:0")
+    cus = torch.cuda.get_device_properties(device).multi_processor_count
+    sink = torch.empty(cus * 8 * 256, dtype=torch.float32, device=device)
+    stream = _stream(device)
+
+    def timed(fn, reps=3):
+        fn()
+        best = float("inf")
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        return best
+
+    out = {}
+    for kind, name, iters in ((0, "fp16_tflops", 2000), (1, "fp8_tflops", 2000), (2, "fp64_tflops", 4000)):
+        best = 0.0
+        for mult in (2, 4, 8):
+            blocks = cus * mult
+            ms = timed(lambda: call("lthip_probe_mfma", kind, blocks, iters, sink.data_ptr(), stream))
+            best = max(best, lib().lthip_probe_mfma_flops(kind, blocks, iters) / ms / 1e9)
+        out[name] = round(best, 1)
+    n = read_bytes // 4
+    x = torch.empty(n, dtype=torch.float32, device=device)
+    fill_uniform(x, 0.0, 1.0, 1, 99)
+    best = 0.0
+    for mult in (2, 4, 8):
+        ms = timed(lambda: call("lthip_probe_read_f32", x.data_ptr(), n, sink.data_ptr(), cus * mult, stream))
+        best = max(best, n * 4 / ms / 1e6)
+    out["hbm_read_gbs"] = round(best, 1)
+    y = torch.empty_like(x)
+    ms = timed(lambda: copy_f32(y, x))
+    out["hbm_copy_gbs"] = round(2 * n * 4 / ms / 1e6, 1)
+    del x, y
+    return out

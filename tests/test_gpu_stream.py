@@ -79,3 +79,24 @@ def test_sdot_basis_vector(dev):
     e = torch.zeros_like(x)
     e[77777] = 1.0
     assert R.sdot(x, e).item() == x[77777].item()
+
+
+def test_read_probe_reads_every_word(dev):
+    # the XOR of the per-thread XORs is the XOR of every input word: each
+    # element is read exactly once
+    from libthom_amd._lib import call
+    n = (1 << 20) + 4 * 37
+    x = torch.empty(n, dtype=torch.float32, device=dev)
+    R.fill_uniform(x, -1, 1, 5, 6)
+    blocks = 64
+    sink = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+    call("lthip_probe_read_f32", x.data_ptr(), n, sink.data_ptr(), blocks, torch.cuda.current_stream(dev).cuda_stream)
+    got = np.bitwise_xor.reduce(sink.cpu().numpy().view(np.uint32))
+    want = np.bitwise_xor.reduce(x.cpu().numpy().view(np.uint32))
+    assert got == want
+
+
+def test_measured_peaks_are_plausible(dev):
+    p = R.measured_peaks(dev, read_bytes=256 << 20)
+    assert p["fp8_tflops"] > p["fp16_tflops"] > 100 * p["fp64_tflops"] / 10 > 1
+    assert 500 < p["hbm_read_gbs"] and 500 < p["hbm_copy_gbs"]
