@@ -8,8 +8,12 @@
 // free); B is kept K-major.  Tiles are visited in groups of 8 tile-rows and the
 // block ids are remapped so each XCD works on a contiguous run of tiles.
 //
+// Shapes that are tile multiples (M, N % 128, K % 16, 16-byte aligned) take
+// the pipelined kernel below instead: both operands reach LDS by DMA through
+// a ring of 4 slabs, as in gemm_core2.hpp.
+//
 // Reference: dgemm_final.cu:65-213 (64x128x16 tiles, mma.m8n8k4, cp.async).
-#include "lt_common.hpp"
+#include "gemm_core2.hpp"
 
 namespace lt {
 namespace {
@@ -133,6 +137,159 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(const double* __restrict_
             }
 }
 
+// ---- pipelined kernel --------------------------------------------------------
+// A tile [128][16] doubles: 16-byte chunk c (k 2c, 2c+1) of row r at
+// c ^ ((r >> 1) & 7); B tile [16][128] doubles: chunk c of row k at
+// c ^ 8 (k & 1).  A fragment reads (16 rows, one k, 16 lanes) and B fragment
+// reads (16 columns of two rows, 32 lanes) then hit 64 distinct banks.
+constexpr int kStages2 = 4;
+constexpr int kA2Bytes = kBM * kBK * 8, kB2Bytes = kBK * kBN * 8, kStage2Bytes = kA2Bytes + kB2Bytes;
+constexpr int kWaves2 = 8, kWGN2 = 4;                      // 2 x 4 waves, 64 x 32 each
+constexpr int kWN2 = kBN / kWGN2, kTN2 = kWN2 / 16;
+constexpr int kA2Instr = kA2Bytes / (1024 * kWaves2), kB2Instr = kB2Bytes / (1024 * kWaves2);
+constexpr int kLoads2 = kA2Instr + kB2Instr;
+
+__device__ __forceinline__ int swzA64(int r) { return (r >> 1) & 7; }
+__device__ __forceinline__ int swzB64(int k) { return 8 * (k & 1); }
+
+__global__ __launch_bounds__(64 * kWaves2) void dgemm2_kernel(const double* __restrict__ A, const double* __restrict__ B,
+                                                     double* __restrict__ C, int M, int N, int K, double alpha,
+                                                     double beta, int tiles_m, int tiles_n) {
+    __shared__ __align__(16) char smem[kStages2 * kStage2Bytes];
+    int tm, tn;
+    tile_coords(xcd_remap(blockIdx.x, gridDim.x), tiles_m, tiles_n, tm, tn);
+    const int row0 = tm * kBM, col0 = tn * kBN;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = (wave / kWGN2) * 64, wn = (wave % kWGN2) * kWN2;
+
+    // per-lane DMA sources: A instruction i covers rows 8 inst .. +7 (8 chunks
+    // each), B instruction i one row of 64 chunks
+    const double* asrc[kA2Instr];
+    const double* bsrc[kB2Instr];
+#pragma unroll
+    for (int i = 0; i < kA2Instr; i++) {
+        const int inst = wave * kA2Instr + i;
+        const int r = inst * 8 + lane / 8;
+        const int c = (lane % 8) ^ swzA64(r);
+        asrc[i] = A + (size_t)(row0 + r) * K + 2 * c;
+    }
+#pragma unroll
+    for (int i = 0; i < kB2Instr; i++) {
+        const int k = wave * kB2Instr + i;
+        const int c = lane ^ swzB64(k);
+        bsrc[i] = B + (size_t)k * N + col0 + 2 * c;
+    }
+    const size_t b_step = (size_t)kBK * N;
+    auto issue = [&](auto q0c, auto q1c, int stage, int slab) {
+        constexpr int Q0 = decltype(q0c)::value, Q1 = decltype(q1c)::value;
+        char* sa = smem + stage * kStage2Bytes;
+        char* sb = sa + kA2Bytes;
+#pragma unroll
+        for (int q = Q0; q < Q1; q++) {
+            if (q < kA2Instr)
+                __builtin_amdgcn_global_load_lds(asrc[q] + (size_t)slab * kBK,
+                                                 (lds_void_t*)(lds_char_t*)(sa + (wave * kA2Instr + q) * 1024), 16, 0,
+                                                 0);
+            else
+                __builtin_amdgcn_global_load_lds(
+                    bsrc[q - kA2Instr] + (size_t)slab * b_step,
+                    (lds_void_t*)(lds_char_t*)(sb + (wave * kB2Instr + q - kA2Instr) * 1024), 16, 0, 0);
+        }
+    };
+    const int fr = lane & 15, fk = lane >> 4;
+    struct Frag {
+        double a[4], b[kTN2];
+    };
+    auto load_frag = [&](Frag& f, const char* sa, const char* sb, int kk) {
+        const int k = 4 * kk + fk;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int r = wm + 16 * i + fr;
+            f.a[i] = *reinterpret_cast<const double*>(sa + r * (kBK * 8) + (((k >> 1) ^ swzA64(r)) << 4) + (k & 1) * 8);
+        }
+#pragma unroll
+        for (int j = 0; j < kTN2; j++) {
+            const int c = wn + 16 * j + fr;
+            f.b[j] = *reinterpret_cast<const double*>(sb + k * (kBN * 8) + (((c >> 1) ^ swzB64(k)) << 4) + (c & 1) * 8);
+        }
+    };
+
+    v4d acc[4][kTN2];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < kTN2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    constexpr int KSTEPS = kBK / 4;
+    const int nslabs = K / kBK;
+    if (nslabs > 0) {
+#pragma unroll
+        for (int p = 0; p < kStages2 - 1; p++) issue(int_c<0>{}, int_c<kLoads2>{}, p, p < nslabs ? p : nslabs - 1);
+        ring_barrier<kLoads2 * (kStages2 - 2)>();
+        Frag fr2[2];
+        load_frag(fr2[0], smem, smem + kA2Bytes, 0);
+        int stage = 0, fill_stage = kStages2 - 1;
+        for (int s = 0; s < nslabs; s++) {
+            const char* sa = smem + stage * kStage2Bytes;
+            const char* sb = sa + kA2Bytes;
+            const int fill = s + kStages2 - 1 < nslabs ? s + kStages2 - 1 : nslabs - 1;
+            __builtin_amdgcn_sched_barrier(0);
+            static_for<0, KSTEPS - 1>([&](auto kkc) {
+                constexpr int kk = decltype(kkc)::value;
+                load_frag(fr2[(kk + 1) & 1], sa, sb, kk + 1);
+                issue(int_c<kk * kLoads2 / KSTEPS>{}, int_c<(kk + 1) * kLoads2 / KSTEPS>{}, fill_stage, fill);
+                const Frag& f = fr2[kk & 1];
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < kTN2; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+                interleave<4 * kTN2, 4 + kTN2, (kk + 1) * kLoads2 / KSTEPS - kk * kLoads2 / KSTEPS>();
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            {
+                constexpr int KL = KSTEPS - 1;
+                const Frag& f = fr2[KL & 1];
+                issue(int_c<KL * kLoads2 / KSTEPS>{}, int_c<kLoads2>{}, fill_stage, fill);
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < kTN2; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+                interleave<2 * kTN2, 0, kLoads2 - KL * kLoads2 / KSTEPS>();
+                __builtin_amdgcn_sched_barrier(0);
+                ring_barrier<kLoads2 * (kStages2 - 2)>();
+                stage = stage + 1 == kStages2 ? 0 : stage + 1;
+                fill_stage = fill_stage + 1 == kStages2 ? 0 : fill_stage + 1;
+                const char* na = smem + stage * kStage2Bytes;
+                load_frag(fr2[0], na, na + kA2Bytes, 0);   // KSTEPS is even
+#pragma unroll
+                for (int i = 2; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < kTN2; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+                interleave<2 * kTN2, 4 + kTN2, 0>();
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        wait_vmcnt<0>();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < kTN2; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = row0 + wm + 16 * i + 4 * r + fk, col = col0 + wn + 16 * j + fr;
+                double* cp = C + (size_t)row * N + col;
+                double v = alpha * acc[i][j][r];
+                if (beta != 0.0) v = fma(beta, *cp, v);
+                *cp = v;
+            }
+}
+
 }  // namespace
 }  // namespace lt
 
@@ -149,8 +306,8 @@ extern "C" int lthip_dgemm_f64(const double* A, const double* B, double* C, size
     const bool fast = (m % kBM == 0) && (n % kBN == 0) && (k % kBK == 0) && (reinterpret_cast<uintptr_t>(A) % 16 == 0) &&
                       (reinterpret_cast<uintptr_t>(B) % 16 == 0);
     if (fast)
-        hipLaunchKernelGGL((dgemm_kernel<true>), dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), A, B, C, m, n,
-                           k, alpha, beta, tiles_m, tiles_n);
+        hipLaunchKernelGGL(dgemm2_kernel, dim3((unsigned)blocks), dim3(64 * kWaves2), 0, as_stream(stream), A, B, C, m,
+                           n, k, alpha, beta, tiles_m, tiles_n);
     else
         hipLaunchKernelGGL((dgemm_kernel<false>), dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), A, B, C, m,
                            n, k, alpha, beta, tiles_m, tiles_n);

@@ -1,8 +1,8 @@
 // Peak probes: the MFMA throughput and HBM read bandwidth this device
 // actually delivers, the denominators of "fraction of measured peak".
 //
-// MFMA probe: every wave runs kChains independent accumulator chains of one
-// MFMA instruction for `iters` rounds on register operands (no memory
+// MFMA probe: every wave runs independent accumulator chains of one MFMA
+// instruction for `iters` rounds on register operands (no memory
 // traffic); one value per thread is written at the end so nothing is
 // optimised away.  Read probe: a grid-stride float4 stream whose per-thread
 // XOR of the loaded bits is written out.
@@ -61,21 +61,23 @@ __global__ __launch_bounds__(256) void probe_f8(float* sink, int iters) {
     sink[blockIdx.x * 256 + threadIdx.x] = s;
 }
 
-// FP64 MFMA has the longest latency: twice the chains
-constexpr int kChains64 = 2 * kChains;
+// FP64: one operand pair, kChains accumulators.  The compiler keeps these in
+// VGPRs; wider variants (several operand pairs, 16 accumulators) get their
+// accumulators copied through AGPRs every round and measure lower.  A GEMM
+// with memory traffic can still run its FP64 MFMAs faster than this loop.
 __global__ __launch_bounds__(256) void probe_f64(float* sink, int iters) {
-    v4d acc[kChains64];
+    v4d acc[kChains];
 #pragma unroll
-    for (int c = 0; c < kChains64; c++)
+    for (int c = 0; c < kChains; c++)
 #pragma unroll
         for (int r = 0; r < 4; r++) acc[c][r] = 0.0;
     const double a = threadIdx.x * 1e-3, b = 0.999;
     for (int it = 0; it < iters; it++)
 #pragma unroll
-        for (int c = 0; c < kChains64; c++) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+        for (int c = 0; c < kChains; c++) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
     double s = 0.0;
 #pragma unroll
-    for (int c = 0; c < kChains64; c++)
+    for (int c = 0; c < kChains; c++)
 #pragma unroll
         for (int r = 0; r < 4; r++) s += acc[c][r];
     sink[blockIdx.x * 256 + threadIdx.x] = (float)s;
@@ -108,7 +110,7 @@ extern "C" {
 double lthip_probe_mfma_flops(int kind, int blocks, int iters) {
     const double per_wave_round = kind == 0   ? 8.0 * 32 * 32 * 16 * 2
                                   : kind == 1 ? 4.0 * 32 * 32 * 64 * 2
-                                  : kind == 2 ? 16.0 * 16 * 16 * 4 * 2
+                                  : kind == 2 ? (double)lt::kChains * 16 * 16 * 4 * 2
                                               : 0.0;
     return per_wave_round * 4.0 * (double)blocks * (double)iters;
 }

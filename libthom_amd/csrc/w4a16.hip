@@ -53,11 +53,12 @@ constexpr int kStreamCols = 64;
 // U k-steps per wave are loaded before any is used: every load is
 // unconditional (out-of-range indices are clamped to a valid address and the
 // activation operand is zeroed instead), so all of them are in flight at once.
-template <int MT, int U>
-__global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restrict__ A, BW4Params p,
-                                                           __half* __restrict__ C, int M, int N, int K, int splits,
-                                                           float* __restrict__ partial, unsigned* __restrict__ counters) {
-    __shared__ float red[3][16][64];
+template <int MT, int U, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void w4a16_stream_kernel(const __half* __restrict__ A, BW4Params p,
+                                                                  __half* __restrict__ C, int M, int N, int K,
+                                                                  int splits, float* __restrict__ partial,
+                                                                  unsigned* __restrict__ counters) {
+    __shared__ float red[WAVES - 1][16][64];
     __shared__ int last_flag;
     const int ntile = blockIdx.x, split = blockIdx.y;
     const int n0 = ntile * kStreamCols;
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restr
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[mt][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-    for (int ks = ks0 + wave; ks < ks1; ks += 4 * U) {
+    for (int ks = ks0 + wave; ks < ks1; ks += WAVES * U) {
         uint4 w[U];
         uint2 s[U];
         uint32_t z[U];
@@ -84,8 +85,8 @@ __global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restr
         bool kv[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int kp = (ks + 4 * u) * 4 + kb;
-            kv[u] = (ks + 4 * u < ks1) && (kp < KP);
+            const int kp = (ks + WAVES * u) * 4 + kb;
+            kv[u] = (ks + WAVES * u < ks1) && (kp < KP);
             const int kpc = kv[u] ? kp : 0;
             const int g = kpc * 8 / p.group;
             w[u] = ld_nt(reinterpret_cast<const uint4*>(p.W + (long)kpc * N + ncolc));
@@ -138,9 +139,12 @@ __global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restr
 #pragma unroll
             for (int j = 0; j < 4; j++)
 #pragma unroll
-                for (int i = 0; i < 4; i++)
-                    acc[mt][j][i] = ((acc[mt][j][i] + red[0][j * 4 + i][lane]) + red[1][j * 4 + i][lane]) +
-                                    red[2][j * 4 + i][lane];
+                for (int i = 0; i < 4; i++) {
+                    float v = acc[mt][j][i];
+#pragma unroll
+                    for (int w = 0; w < WAVES - 1; w++) v += red[w][j * 4 + i][lane];
+                    acc[mt][j][i] = v;
+                }
         }
         __syncthreads();
     }
@@ -184,9 +188,9 @@ __global__ __launch_bounds__(256) void w4a16_stream_kernel(const __half* __restr
     if (threadIdx.x == 0) last_flag = (take_ticket(&counters[ntile]) == (unsigned)splits - 1);
     __syncthreads();
     if (!last_flag) return;
-    // the last block: 256 threads sum the 64-column x min(M, 16 MT)-row tile, 4 columns each
+    // the last block sums the 64-column x min(M, 16 MT)-row tile, 4 columns per thread
     const int rows = M < 16 * MT ? M : 16 * MT;
-    for (int e = threadIdx.x; e < rows * (kStreamCols / 4); e += 256) {
+    for (int e = threadIdx.x; e < rows * (kStreamCols / 4); e += 64 * WAVES) {
         const int m = e / (kStreamCols / 4), n = n0 + 4 * (e % (kStreamCols / 4));
         if (n >= N) continue;
         // splits summed in order, four splits' loads in flight at a time
@@ -330,17 +334,11 @@ void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_
     lthip_w4a16_config c = cfg ? *cfg : lthip_w4a16_config{0, 0, 0, 0};
     if (c.kind == 0) {
         if (M <= 64) {
-            // split K (FP32 partials) only for the thinnest problems: for a
-            // few rows the weight stream needs more blocks than the column
-            // tiles give, from 16 rows on the partial traffic costs more
+            // no split-K: the launch runs up to 16 waves per column tile, one
+            // unrolled round of weight loads each, and the FP32 partials of a
+            // split cost more than the extra blocks gain
             c.kind = 1;
-            const long ntiles = (long)((N + kStreamCols - 1) / kStreamCols);
-            const long ksteps = (long)((K / 8 + 3) / 4);
-            long split = M <= 8 ? (2L * cu_count() + ntiles - 1) / ntiles : 1;
-            if (split > 4) split = 4;
-            if (split > ksteps / 4) split = ksteps / 4;
-            if (split < 1) split = 1;
-            c.split_k = (int)split;
+            c.split_k = 1;
             c.tile_n = kStreamCols;
             c.tile_m = 16;
         } else if (N % 32 == 0) {
@@ -424,18 +422,25 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
         const __half* a = reinterpret_cast<const __half*>(A);
         __half* cc = reinterpret_cast<__half*>(C);
         const int mt = (m + 15) / 16;
-        if (mt <= 1)
-            hipLaunchKernelGGL((w4a16_stream_kernel<1, 4>), grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial,
-                               counters);
-        else if (mt <= 2)
-            hipLaunchKernelGGL((w4a16_stream_kernel<2, 4>), grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial,
-                               counters);
-        else if (mt <= 4)
-            hipLaunchKernelGGL((w4a16_stream_kernel<4, 2>), grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial,
-                               counters);
-        else
-            hipLaunchKernelGGL((w4a16_stream_kernel<8, 2>), grid, dim3(256), 0, s, a, p, cc, m, n, k, splits, partial,
-                               counters);
+        // waves per block: enough that one unrolled round of loads covers the
+        // split's k-steps (more bytes in flight per CU), 4 to 16
+        const int unroll = mt <= 1 ? 8 : (mt <= 2 ? 4 : 2);
+        const int per_split = (ksteps + splits - 1) / splits;
+        const int want = (per_split + unroll - 1) / unroll;
+        const int waves = want >= 16 && mt <= 2 ? 16 : (want >= 8 ? 8 : 4);
+#define LT_W4S(MT_, U_, W_)                                                                                       \
+    hipLaunchKernelGGL((w4a16_stream_kernel<MT_, U_, W_>), grid, dim3(64 * W_), 0, s, a, p, cc, m, n, k, splits, \
+                       partial, counters)
+        if (mt <= 1) {
+            if (waves == 16) LT_W4S(1, 8, 16); else if (waves == 8) LT_W4S(1, 8, 8); else LT_W4S(1, 8, 4);
+        } else if (mt <= 2) {
+            if (waves == 16) LT_W4S(2, 4, 16); else if (waves == 8) LT_W4S(2, 4, 8); else LT_W4S(2, 4, 4);
+        } else if (mt <= 4) {
+            if (waves == 8) LT_W4S(4, 2, 8); else LT_W4S(4, 2, 4);
+        } else {
+            if (waves == 8) LT_W4S(8, 2, 8); else LT_W4S(8, 2, 4);
+        }
+#undef LT_W4S
         LT_TRY_LAUNCH("w4a16 stream launch");
         return LTHIP_OK;
     }

@@ -308,7 +308,7 @@ def measured_peaks(device=None, read_bytes: int = 2 << 30) -> dict:
     caches)."""
     device = torch.device(device or "cuda:0")
     cus = torch.cuda.get_device_properties(device).multi_processor_count
-    sink = torch.empty(cus * 8 * 256, dtype=torch.float32, device=device)
+    sink = torch.empty(cus * 16 * 256, dtype=torch.float32, device=device)
     stream = _stream(device)
 
     def timed(fn, reps=3):
@@ -324,9 +324,9 @@ def measured_peaks(device=None, read_bytes: int = 2 << 30) -> dict:
         return best
 
     out = {}
-    for kind, name, iters in ((0, "fp16_tflops", 2000), (1, "fp8_tflops", 2000), (2, "fp64_tflops", 4000)):
+    for kind, name, iters in ((0, "fp16_tflops", 2000), (1, "fp8_tflops", 2000), (2, "fp64_tflops", 2000)):
         best = 0.0
-        for mult in (2, 4, 8):
+        for mult in (2, 4, 8, 16):
             blocks = cus * mult
             ms = timed(lambda: call("lthip_probe_mfma", kind, blocks, iters, sink.data_ptr(), stream))
             best = max(best, lib().lthip_probe_mfma_flops(kind, blocks, iters) / ms / 1e9)
