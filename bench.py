@@ -31,6 +31,7 @@ FP64_PEAK = 78.6      # TFLOP/s dense MFMA
 FP16_PEAK = 2500.0    # TFLOP/s dense MFMA
 FP8_PEAK = 5000.0
 M_SG, N_SG = 16384, 8192
+OZ_SLICES = (4, 6, 8, 10, 11, 12, 13)   # Ozaki slice counts in the emulated-vs-native study
 SG_BYTES = 4 * (M_SG * N_SG + N_SG + M_SG)
 
 
@@ -153,7 +154,7 @@ def bench_routines(torch, R, timer, dev, quick):
         tf = 2 * n ** 3 / ms / 1e9
         row = {"native": {"ms": round(ms, 4), "TFLOP/s": round(tf, 2), "frac_fp64_mfma": round(tf / FP64_PEAK, 4),
                           "max_rel_err": err(C)}}
-        for S in (4, 6, 8, 9, 10):
+        for S in OZ_SLICES:
             Co = torch.empty_like(C)
             R.dgemm_ozaki(A, B, S, Co)
             mso = timer.per_launch_ms(lambda: R.dgemm_ozaki(A, B, S, Co), max(3, it // 4))
@@ -162,10 +163,11 @@ def bench_routines(torch, R, timer, dev, quick):
             row[f"ozaki{S}"] = {"ms": round(mso, 4), "TFLOP/s_fp64_equiv": round(tfo, 2),
                                 "fp8_TFLOP/s": round(tfo * pairs, 1),
                                 "frac_fp8_mfma": round(tfo * pairs / FP8_PEAK, 4), "max_rel_err": err(Co)}
-        eq = [S for S in (4, 6, 8, 9, 10) if row[f"ozaki{S}"]["max_rel_err"] <= row["native"]["max_rel_err"]]
+        eq = [S for S in OZ_SLICES if row[f"ozaki{S}"]["max_rel_err"] <= row["native"]["max_rel_err"]]
         row["equal_error"] = ({"slices": eq[0], "emulated_TFLOP/s": row[f"ozaki{eq[0]}"]["TFLOP/s_fp64_equiv"],
                                "native_TFLOP/s": row["native"]["TFLOP/s"]} if eq else
-                              {"slices": None, "note": "no slice count up to 10 reaches the native error"})
+                              {"slices": None,
+                               "note": f"no slice count up to {OZ_SLICES[-1]} reaches the native error"})
         study[label] = row
         del A, B, C, hi, lo, scale
         torch.cuda.empty_cache()

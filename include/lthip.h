@@ -111,8 +111,10 @@ int lthip_dgemm_f64(const double *A, const double *B, double *C, size_t M, size_
  *      scaled by powers of two and cut into `slices` signed base-32 digits
  *      (E4M3-exact, |d| <= 16); slice products run on
  *      v_mfma_scale_f32_16x16x128_f8f6f4 with exact FP32 level sums and are
- *      recombined in FP64.  2 <= slices <= 10, K % 16 == 0 and
- *      slices * 256 * K < 2^24 (lthip_ozaki_max_slices(K) gives the limit). */
+ *      recombined in FP64.  2 <= slices <= 10 (<= 15 when M % 256 == 0,
+ *      N % 128 == 0 and K % 256 == 0: one FP8 GEMM per level), K % 16 == 0 and
+ *      slices * 256 * K < 2^24 (lthip_ozaki_max_slices(K) gives the limit for
+ *      any shape, at most 10). */
 int lthip_ozaki_max_slices(size_t K);
 size_t lthip_dgemm_ozaki_workspace_size(size_t M, size_t N, size_t K, int slices);
 int lthip_dgemm_ozaki(const double *A, const double *B, double *C, size_t M, size_t N, size_t K,

@@ -1,6 +1,7 @@
 // dgemm routine driver (the reference's routines/dgemm/dgemm.cpp): C = alpha*A*B
 // + beta*C in FP64.  Implementations: "hip" (native FP64 MFMA) and
-// "ozaki<S>" (split-precision on FP8 MFMA with S digits, S = 4..10).
+// "ozaki<S>" (split-precision on FP8 MFMA with S digits, S = 4..13; past 10
+// only for shapes with M % 256 == N % 128 == K % 256 == 0).
 // Inputs: N(0,1), or with a trailing "scaled" token rows of A and columns of B
 // multiplied by 2^u, u uniform in [-20, 20].  The check compares sampled
 // entries with a compensated (Dot2) FP64 dot product on the host and reports
@@ -47,6 +48,7 @@ static const std::unordered_map<std::string, DgemmImpl> dgemmImpls = {
 #if LIB_HAS(DGEMM_OZAKI)
     {"ozaki4", dgemm_ozaki<4>}, {"ozaki5", dgemm_ozaki<5>}, {"ozaki6", dgemm_ozaki<6>}, {"ozaki7", dgemm_ozaki<7>},
     {"ozaki8", dgemm_ozaki<8>}, {"ozaki9", dgemm_ozaki<9>}, {"ozaki10", dgemm_ozaki<10>},
+    {"ozaki11", dgemm_ozaki<11>}, {"ozaki12", dgemm_ozaki<12>}, {"ozaki13", dgemm_ozaki<13>},
 #endif
 };
 

@@ -16,9 +16,16 @@
 // from the last level, scaled by 2^(e_i + f_j - 8), then alpha/beta are
 // applied -- exactly the order of the oracle's orc_ozaki_dgemm.
 //
+// Level GEMMs (M % 256 == N % 128 == K % 256 == 0): level l is one FP8 GEMM
+// over K_eff = (l + 1) K, the slices concatenated along k (A_0..A_l against
+// B_l..B_0), on a 256x128 tile fed by a DMA ring (gemm_core2.hpp's scheme);
+// its FP32 sums (the same exact integers) go to the workspace and a combine
+// kernel applies the same FP64 Horner recombination.  Other shapes use the
+// all-levels-in-registers kernel below.
+//
 // This routine has no counterpart in the reference; it sits beside the
 // native FP64 path (dgemm_final.cu) as the emulated alternative.
-#include "lt_common.hpp"
+#include "gemm_core2.hpp"
 
 namespace lt {
 namespace {
@@ -193,6 +200,162 @@ __global__ __launch_bounds__(256) void ozaki_fp8_kernel(const uint8_t* __restric
             }
 }
 
+// ---- level GEMMs ------------------------------------------------------------
+constexpr int kLvBM = 256, kLvBN = 128, kLvBK = 128, kLvStages = 3;
+constexpr int kLvA = kLvBM * kLvBK, kLvB = kLvBN * kLvBK, kLvStage = kLvA + kLvB;
+constexpr int kLvAInstr = kLvA / 4096, kLvBInstr = kLvB / 4096, kLvLoads = kLvAInstr + kLvBInstr;
+constexpr int kLvTM = 8, kLvTN = 4;   // 16x16 MFMA tiles per wave: 128 x 64
+
+// 128-byte rows, 16-byte chunk c of row r at c ^ ((r >> 1) & 7): the 16 rows a
+// fragment read touches fall on 64 distinct banks
+__device__ __forceinline__ int lv_swz(int r) { return (r >> 1) & 7; }
+
+struct LvFrag {
+    v8i a[kLvTM], b[kLvTN];
+};
+
+__device__ __forceinline__ void lv_load(LvFrag& f, const char* sa, const char* sb, int wm, int wn) {
+    const int lane = threadIdx.x & 63, fr = lane & 15, fk = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < kLvTM; i++) {
+        const int r = wm + 16 * i + fr;
+        const uint4 lo = *reinterpret_cast<const uint4*>(sa + r * kLvBK + (((2 * fk) ^ lv_swz(r)) << 4));
+        const uint4 hi = *reinterpret_cast<const uint4*>(sa + r * kLvBK + (((2 * fk + 1) ^ lv_swz(r)) << 4));
+        f.a[i] = v8i{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+    }
+#pragma unroll
+    for (int j = 0; j < kLvTN; j++) {
+        const int r = wn + 16 * j + fr;
+        const uint4 lo = *reinterpret_cast<const uint4*>(sb + r * kLvBK + (((2 * fk) ^ lv_swz(r)) << 4));
+        const uint4 hi = *reinterpret_cast<const uint4*>(sb + r * kLvBK + (((2 * fk + 1) ^ lv_swz(r)) << 4));
+        f.b[j] = v8i{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+    }
+}
+
+template <int I0, int I1>
+__device__ __forceinline__ void lv_mfma(v4f (&acc)[kLvTM][kLvTN], const LvFrag& f) {
+#pragma unroll
+    for (int i = I0; i < I1; i++)
+#pragma unroll
+        for (int j = 0; j < kLvTN; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(f.a[i], f.b[j], acc[i][j], 0, 0, 0, 0, 0, 0);
+}
+
+// blockIdx.y = S-1-l: the longest levels start first.  lv[l][M][N] = level sums.
+__global__ __launch_bounds__(256) void ozaki_level_kernel(const uint8_t* __restrict__ Ad,
+                                                          const uint8_t* __restrict__ Bd, float* __restrict__ lv,
+                                                          int M, int N, int K, int S, int tiles_m, int tiles_n) {
+    __shared__ __align__(16) char smem[kLvStages * kLvStage];
+    const int l = S - 1 - (int)blockIdx.y;
+    int tm, tn;
+    oz_tile_coords(xcd_remap(blockIdx.x, gridDim.x), tiles_m, tiles_n, tm, tn);
+    const int row0 = tm * kLvBM, col0 = tn * kLvBN;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = (wave >> 1) * 128, wn = (wave & 1) * 64;
+    const long sa = (long)M * K, sb = (long)N * K;
+    const int kc = K / kLvBK;           // slabs per slice pair
+    const int n = (l + 1) * kc;         // slabs of this level
+
+    // per-lane source offsets within a slice (row * K + chunk), 8 rows per instruction
+    long aoff[kLvAInstr], boff[kLvBInstr];
+#pragma unroll
+    for (int i = 0; i < kLvAInstr; i++) {
+        const int r = (wave * kLvAInstr + i) * 8 + lane / 8;
+        aoff[i] = (long)(row0 + r) * K + (((lane % 8) ^ lv_swz(r)) << 4);
+    }
+#pragma unroll
+    for (int i = 0; i < kLvBInstr; i++) {
+        const int r = (wave * kLvBInstr + i) * 8 + lane / 8;
+        boff[i] = (long)(col0 + r) * K + (((lane % 8) ^ lv_swz(r)) << 4);
+    }
+    // slab t: pair p = t / kc (A slice p, B slice l - p), k offset (t % kc) * 128
+    auto issue = [&](auto q0c, auto q1c, int stage, int t) {
+        constexpr int Q0 = decltype(q0c)::value, Q1 = decltype(q1c)::value;
+        const int p = t / kc;
+        const long koff = (long)(t - p * kc) * kLvBK;
+        const uint8_t* ab = Ad + p * sa + koff;
+        const uint8_t* bb = Bd + (l - p) * sb + koff;
+        char* ds = smem + stage * kLvStage;
+#pragma unroll
+        for (int q = Q0; q < Q1; q++) {
+            if (q < kLvAInstr)
+                __builtin_amdgcn_global_load_lds(ab + aoff[q],
+                                                 (lds_void_t*)(lds_char_t*)(ds + (wave * kLvAInstr + q) * 1024), 16,
+                                                 0, 0);
+            else
+                __builtin_amdgcn_global_load_lds(
+                    bb + boff[q - kLvAInstr],
+                    (lds_void_t*)(lds_char_t*)(ds + kLvA + (wave * kLvBInstr + q - kLvAInstr) * 1024), 16, 0, 0);
+        }
+    };
+
+    v4f acc[kLvTM][kLvTN];
+#pragma unroll
+    for (int i = 0; i < kLvTM; i++)
+#pragma unroll
+        for (int j = 0; j < kLvTN; j++) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    // ring of 3 slabs; one slab = one k-step, so the fragments alternate
+    // between two register sets by slab parity (the loop runs slab pairs)
+#pragma unroll
+    for (int q = 0; q < kLvStages - 1; q++) issue(int_c<0>{}, int_c<kLvLoads>{}, q, q < n ? q : n - 1);
+    ring_barrier<kLvLoads * (kLvStages - 2)>();
+    LvFrag f0, f1;
+    lv_load(f0, smem, smem + kLvA, wm, wn);
+    int stage = 0, fill_stage = kLvStages - 1;
+    auto step = [&](int t, const LvFrag& cur, LvFrag& nxt) {
+        const int fill = t + kLvStages - 1 < n ? t + kLvStages - 1 : n - 1;
+        __builtin_amdgcn_sched_barrier(0);
+        issue(int_c<0>{}, int_c<kLvLoads>{}, fill_stage, fill);
+        lv_mfma<0, kLvTM / 2>(acc, cur);
+        interleave<kLvTM / 2 * kLvTN, 0, kLvLoads>();
+        __builtin_amdgcn_sched_barrier(0);
+        ring_barrier<kLvLoads * (kLvStages - 2)>();
+        stage = stage + 1 == kLvStages ? 0 : stage + 1;
+        fill_stage = fill_stage + 1 == kLvStages ? 0 : fill_stage + 1;
+        const char* na = smem + stage * kLvStage;
+        lv_load(nxt, na, na + kLvA, wm, wn);
+        lv_mfma<kLvTM / 2, kLvTM>(acc, cur);
+        interleave<kLvTM / 2 * kLvTN, kLvTM * 2 + kLvTN * 2, 0>();
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    for (int t = 0; t < n; t += 2) {   // n is even (K % 256 == 0)
+        step(t, f0, f1);
+        step(t + 1, f1, f0);
+    }
+    wait_vmcnt<0>();
+
+    // C fragment of a 16x16 f32 tile: register r of lane x is row 4(x/16) + r, column x%16
+    float* out = lv + (long)l * M * N;
+    const int fr = lane & 15, fk = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < kLvTM; i++)
+#pragma unroll
+        for (int j = 0; j < kLvTN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = row0 + wm + 16 * i + 4 * fk + r, col = col0 + wn + 16 * j + fr;
+                out[(long)row * N + col] = acc[i][j][r];
+            }
+}
+
+// C = alpha 2^(e_i + f_j - 8) Horner_l(level sums, 2^-5) + beta C, in the
+// operation order of the all-levels kernel and of the oracle
+__global__ __launch_bounds__(256) void ozaki_combine_kernel(const float* __restrict__ lv, const int32_t* __restrict__ ea,
+                                                            const int32_t* __restrict__ eb, double* __restrict__ C,
+                                                            int M, int N, int S, double alpha, double beta) {
+    const long total = (long)M * N, slice = total;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const int row = (int)(e / N), col = (int)(e % N);
+        double h = (double)lv[(long)(S - 1) * slice + e];
+        for (int l = S - 2; l >= 0; l--) h = __dadd_rn((double)lv[(long)l * slice + e], __dmul_rn(h, 0x1p-5));
+        double v = __dmul_rn(alpha, ldexp(h, ea[row] + eb[col] - 8));
+        if (beta != 0.0) v = __dadd_rn(v, __dmul_rn(beta, C[e]));
+        C[e] = v;
+    }
+}
+
 // ---- compensated FP64 GEMM (Dot2: TwoProduct by fma + TwoSum) ---------------
 // One output per thread, k ascending, exactly the algorithm of the oracle's
 // orc_dgemm_dot2_entries.  A 16x16 thread block shares 16x16 operand tiles.
@@ -232,9 +395,14 @@ __global__ __launch_bounds__(256) void dot2_kernel(const double* __restrict__ A,
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 struct OzLayout {
-    size_t ad, bd, ea, eb, bt, total;
+    size_t ad, bd, ea, eb, bt, lv, total;
 };
 
+bool oz_levels_fit(size_t M, size_t N, size_t K) {
+    return M % kLvBM == 0 && N % kLvBN == 0 && K % (2 * kLvBK) == 0 && K > 0;   // an even slab count
+}
+
+// level sums [S][M][N] FP32 only when the level GEMMs run
 OzLayout oz_layout(size_t M, size_t N, size_t K, int S) {
     OzLayout l;
     l.ad = 0;
@@ -242,7 +410,8 @@ OzLayout oz_layout(size_t M, size_t N, size_t K, int S) {
     l.ea = l.bd + align256((size_t)S * N * K);
     l.eb = l.ea + align256(4 * M);
     l.bt = l.eb + align256(4 * N);
-    l.total = l.bt + align256(8 * N * K);
+    l.lv = l.bt + align256(8 * N * K);
+    l.total = l.lv + (oz_levels_fit(M, N, K) ? align256(4 * (size_t)S * M * N) : 0);
     return l;
 }
 
@@ -252,11 +421,15 @@ typedef void (*prod_fn)(const uint8_t*, const uint8_t*, const int32_t*, const in
 
 #define LT_OZ_SPLIT(S) ozaki_split_kernel<S>
 #define LT_OZ_PROD(S) ozaki_fp8_kernel<S>
-const split_fn kSplit[] = {nullptr, nullptr, LT_OZ_SPLIT(2), LT_OZ_SPLIT(3), LT_OZ_SPLIT(4), LT_OZ_SPLIT(5),
-                           LT_OZ_SPLIT(6), LT_OZ_SPLIT(7), LT_OZ_SPLIT(8), LT_OZ_SPLIT(9), LT_OZ_SPLIT(10)};
+const split_fn kSplit[] = {nullptr,         nullptr,         LT_OZ_SPLIT(2),  LT_OZ_SPLIT(3),
+                           LT_OZ_SPLIT(4),  LT_OZ_SPLIT(5),  LT_OZ_SPLIT(6),  LT_OZ_SPLIT(7),
+                           LT_OZ_SPLIT(8),  LT_OZ_SPLIT(9),  LT_OZ_SPLIT(10), LT_OZ_SPLIT(11),
+                           LT_OZ_SPLIT(12), LT_OZ_SPLIT(13), LT_OZ_SPLIT(14), LT_OZ_SPLIT(15)};
 const prod_fn kProd[] = {nullptr, nullptr, LT_OZ_PROD(2), LT_OZ_PROD(3), LT_OZ_PROD(4), LT_OZ_PROD(5),
                          LT_OZ_PROD(6), LT_OZ_PROD(7), LT_OZ_PROD(8), LT_OZ_PROD(9), LT_OZ_PROD(10)};
-constexpr int kMinSlices = 2, kMaxSlices = 10;
+// the all-levels-in-registers kernel stops at 10 slices; the level GEMMs take
+// up to 15 (the exactness bound slices * 256 * K < 2^24 still applies)
+constexpr int kMinSlices = 2, kMaxSlices = 10, kMaxSlicesLevels = 15;
 
 }  // namespace
 }  // namespace lt
@@ -270,14 +443,16 @@ int lthip_ozaki_max_slices(size_t K) {
 }
 
 size_t lthip_dgemm_ozaki_workspace_size(size_t M, size_t N, size_t K, int slices) {
-    if (slices < lt::kMinSlices || slices > lt::kMaxSlices) return 0;
+    const int top = lt::oz_levels_fit(M, N, K) ? lt::kMaxSlicesLevels : lt::kMaxSlices;
+    if (slices < lt::kMinSlices || slices > top) return 0;
     return lt::oz_layout(M, N, K, slices).total;
 }
 
 int lthip_ozaki_split_rows(const double* X, size_t rows, size_t cols, size_t ld, int slices, uint8_t* digits,
                            int32_t* exps, void* stream) {
     using namespace lt;
-    if (slices < kMinSlices || slices > kMaxSlices) return fail(LTHIP_ERR_INVALID_ARG, "ozaki: slices must be 2..10");
+    if (slices < kMinSlices || slices > kMaxSlicesLevels)
+        return fail(LTHIP_ERR_INVALID_ARG, "ozaki: slices must be 2..15");
     if (cols % 4 != 0 || ld < cols || rows > (1u << 30) || cols > (1u << 30))
         return fail(LTHIP_ERR_INVALID_ARG, "ozaki split: cols must be a multiple of 4 and ld >= cols");
     if (rows == 0 || cols == 0) return LTHIP_OK;
@@ -291,8 +466,10 @@ int lthip_ozaki_split_rows(const double* X, size_t rows, size_t cols, size_t ld,
 int lthip_dgemm_ozaki(const double* A, const double* B, double* C, size_t M, size_t N, size_t K, double alpha,
                       double beta, int slices, void* ws, size_t ws_bytes, void* stream) {
     using namespace lt;
-    if (slices < kMinSlices || slices > kMaxSlices) return fail(LTHIP_ERR_INVALID_ARG, "ozaki: slices must be 2..10");
     if (M > (1u << 30) || N > (1u << 30) || K > (1u << 30)) return fail(LTHIP_ERR_INVALID_ARG, "ozaki: size too large");
+    const int top = oz_levels_fit(M, N, K) ? kMaxSlicesLevels : kMaxSlices;
+    if (slices < kMinSlices || slices > top)
+        return fail(LTHIP_ERR_INVALID_ARG, "ozaki: slices must be 2..%d for this shape", top);
     if (K % 16 != 0) return fail(LTHIP_ERR_INVALID_ARG, "ozaki: K must be a multiple of 16");
     if ((double)slices * 256.0 * (double)K >= 16777216.0)
         return fail(LTHIP_ERR_UNSUPPORTED, "ozaki: slices*256*K must stay below 2^24 for exact FP32 level sums");
@@ -318,6 +495,20 @@ int lthip_dgemm_ozaki(const double* A, const double* B, double* C, size_t M, siz
         hipLaunchKernelGGL(kSplit[slices], dim3((unsigned)n), dim3(256), 0, s, bt, k, (long)k, bd, (long)N * K, eb);
         LT_TRY_LAUNCH("ozaki split B launch");
     }  // K == 0: every level sum is 0, whatever the exponents hold
+    if (oz_levels_fit(M, N, K)) {
+        float* lv = reinterpret_cast<float*>(w + l.lv);
+        const int tiles_m = m / kLvBM, tiles_n = n / kLvBN;
+        hipLaunchKernelGGL(ozaki_level_kernel, dim3((unsigned)(tiles_m * tiles_n), (unsigned)slices), dim3(256), 0, s,
+                           ad, bd, lv, m, n, k, slices, tiles_m, tiles_n);
+        LT_TRY_LAUNCH("ozaki level launch");
+        const long total = (long)m * n;
+        long blocks = (total + 255) / 256;
+        if (blocks > (long)cu_count() * 16) blocks = (long)cu_count() * 16;
+        hipLaunchKernelGGL(ozaki_combine_kernel, dim3((unsigned)blocks), dim3(256), 0, s, lv, ea, eb, C, m, n, slices,
+                           alpha, beta);
+        LT_TRY_LAUNCH("ozaki combine launch");
+        return LTHIP_OK;
+    }
     const int tiles_m = (m + kOzTile - 1) / kOzTile, tiles_n = (n + kOzTile - 1) / kOzTile;
     hipLaunchKernelGGL(kProd[slices], dim3((unsigned)(tiles_m * tiles_n)), dim3(256), 0, s, ad, bd, ea, eb, C, m, n, k,
                        alpha, beta, tiles_m, tiles_n);

@@ -37,8 +37,27 @@ def test_split_bit_exact(dev, orc, e4m3_table, S):
 
 
 @pytest.mark.parametrize("S", [2, 4, 7, 10])
-@pytest.mark.parametrize("M,N,K", [(64, 64, 128), (100, 70, 256), (1, 130, 16), (257, 65, 400)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 128), (100, 70, 256), (1, 130, 16), (257, 65, 400),
+                                   (256, 128, 256), (512, 256, 512)])
 def test_ozaki_bit_exact(dev, orc, S, M, N, K):
+    # the last two shapes take the level-GEMM path (M % 256, N % 128, K % 256)
+    _bit_exact(dev, orc, S, M, N, K)
+
+
+@pytest.mark.parametrize("S", [11, 13, 15])
+def test_ozaki_level_path_beyond_ten_slices(dev, orc, S):
+    _bit_exact(dev, orc, S, 256, 256, 512)
+
+
+def test_ozaki_slice_limits(dev):
+    from libthom_amd._lib import LthipError
+    A = torch.zeros(100, 64, dtype=torch.float64, device=dev)
+    B = torch.zeros(64, 100, dtype=torch.float64, device=dev)
+    with pytest.raises(LthipError):
+        R.dgemm_ozaki(A, B, 11)          # only the level-GEMM shapes go past 10
+
+
+def _bit_exact(dev, orc, S, M, N, K):
     rng = np.random.default_rng(M * S + N)
     A = rng.standard_normal((M, K)) * (2.0 ** rng.integers(-20, 21, (M, 1)))
     B = rng.standard_normal((K, N)) * (2.0 ** rng.integers(-20, 21, (1, N)))
