@@ -343,11 +343,11 @@ int lthip_hgemm_grouped_capture(lthip_grouped_plan* plan, void* stream) {
     if (!stream) return fail(LTHIP_ERR_INVALID_ARG, "grouped: graph capture needs a non-null stream");
     hipStream_t s = as_stream(stream);
     if (plan->exec) {
-        hipGraphExecDestroy(plan->exec);
+        (void)hipGraphExecDestroy(plan->exec);
         plan->exec = nullptr;
     }
     if (plan->graph) {
-        hipGraphDestroy(plan->graph);
+        (void)hipGraphDestroy(plan->graph);
         plan->graph = nullptr;
     }
     int st = check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "grouped: begin capture");
@@ -358,7 +358,7 @@ int lthip_hgemm_grouped_capture(lthip_grouped_plan* plan, void* stream) {
     hipGraph_t g = nullptr;
     st = check_hip(hipStreamEndCapture(s, &g), "grouped: end capture");
     if (le != hipSuccess) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return check_hip(le, "grouped: launch during capture");
     }
     if (st != LTHIP_OK) return st;
@@ -375,8 +375,8 @@ int lthip_hgemm_grouped_graph_launch(lthip_grouped_plan* plan, void* stream) {
 
 int lthip_hgemm_grouped_plan_destroy(lthip_grouped_plan* plan) {
     if (!plan) return LTHIP_OK;
-    if (plan->exec) hipGraphExecDestroy(plan->exec);
-    if (plan->graph) hipGraphDestroy(plan->graph);
+    if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
     delete plan;
     return LTHIP_OK;
 }
