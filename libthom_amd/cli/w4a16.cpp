@@ -3,9 +3,10 @@
 // (uniform nibbles, FP16 scales in [0.005, 0.02], per-(group, column) int4
 // zero-points); a trailing "sym" token instead quantizes fp16 weights with the
 // reference's symmetric quantizer (quantize.cpp:8-53, zero-point 8).
-// Implementations: "hip" (size-based choice), "stream" (split-K decode kernel),
-// "tiled" (MFMA tiles, dequantized into LDS), "tuned" (tuning database, else
-// "hip").  `tune w4a16 M N K` sweeps every configuration into the database.
+// Implementations: "hip" (size-based choice), "stream" (decode kernel),
+// "tiled" (MFMA tiles, dequantized into LDS), "pipelined" (DMA-ring MFMA tiles,
+// dequantized in registers), "tuned" (tuning database, else "hip").
+// `tune w4a16 M N K` sweeps every configuration into the database.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -46,14 +47,18 @@ static float run_cfg(const Problem& p, const lthip_w4a16_config* cfg) {
 }
 static float w4a16_hip(const Problem& p) { return run_cfg(p, nullptr); }
 static float w4a16_stream(const Problem& p) {
-    lthip_w4a16_config c{1, 64, 16, 0};
-    const long ntiles = (long)(p.N + 63) / 64;
-    c.split_k = (int)std::max(1L, std::min<long>(8, (4L * 256 + ntiles - 1) / ntiles));
+    lthip_w4a16_config c{1, 64, 16, 1};
     if (p.M > 128) return run_cfg(p, nullptr);
     return run_cfg(p, &c);
 }
 static float w4a16_tiled(const Problem& p) {
     lthip_w4a16_config c{2, 128, 128, 1};
+    return run_cfg(p, &c);
+}
+static float w4a16_pipelined(const Problem& p) {
+    if (p.N % 32 != 0) return run_cfg(p, nullptr);   // the pipelined tiles need N % 32 == 0
+    const int bm = p.M > 128 ? (p.M > 512 ? 256 : 128) : 128, bn = p.M > 128 ? 256 : 128;
+    lthip_w4a16_config c{3, bn, bm, 1};
     return run_cfg(p, &c);
 }
 static float w4a16_tuned(const Problem& p) {
@@ -64,7 +69,8 @@ static float w4a16_tuned(const Problem& p) {
 
 static const std::unordered_map<std::string, W4A16Function> w4a16Impls = {
 #if LIB_HAS(W4A16_HIP)
-    {"hip", w4a16_hip}, {"stream", w4a16_stream}, {"tiled", w4a16_tiled}, {"tuned", w4a16_tuned},
+    {"hip", w4a16_hip},   {"stream", w4a16_stream}, {"tiled", w4a16_tiled}, {"pipelined", w4a16_pipelined},
+    {"tuned", w4a16_tuned},
 #endif
 };
 
@@ -298,6 +304,9 @@ static void tune(int argc, char* argv[]) {
         for (int s : {1, 2, 3, 4, 6, 8, 12, 16}) cands.push_back({1, 64, 16, s});
     for (auto bm_bn : {std::pair<int, int>{64, 64}, {64, 128}, {128, 128}, {128, 256}, {256, 128}, {256, 256}})
         cands.push_back({2, bm_bn.second, bm_bn.first, 1});
+    if (N % 32 == 0)
+        for (auto bm_bn : {std::pair<int, int>{128, 128}, {128, 256}, {256, 128}, {256, 256}})
+            cands.push_back({3, bm_bn.second, bm_bn.first, 1});
     float best = 1e30f;
     lthip_w4a16_config bestc{};
     for (const auto& c : cands) {

@@ -53,10 +53,13 @@ def test_missing_arguments_list_implementations():
     ("dgemm", "hip", 512, 384, 256, 2),
     ("dgemm", "ozaki8", 256, 256, 512, 2),
     ("dgemm", "ozaki10", 256, 256, 512, 2, 1, "scaled"),
+    ("dgemm", "ozaki11", 256, 128, 512, 2),
     ("hgemm", "hip", 1024, 512, 768, 2),
     ("w4a16", "hip", 16, 1024, 1024, 2),
     ("w4a16", "tiled", 192, 512, 1024, 2),
     ("w4a16", "stream", 1, 11008, 4096, 2, 1, "sym"),
+    ("w4a16", "pipelined", 300, 512, 1024, 2),
+    ("w4a16", "pipelined", 2048, 11008, 4096, 2, 1, "sym"),
 ])
 def test_cli_test_mode_passes_checks(args):
     r = run("test", *args)
