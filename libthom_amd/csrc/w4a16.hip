@@ -53,33 +53,36 @@ constexpr int kStreamCols = 64;
 // U k-steps per wave are loaded before any is used: every load is
 // unconditional (out-of-range indices are clamped to a valid address and the
 // activation operand is zeroed instead), so all of them are in flight at once.
-template <int MT, int U, int WAVES>
+// CPL columns per lane (4 or 2): a block covers 16 CPL columns.
+template <int MT, int U, int WAVES, int CPL>
 __global__ __launch_bounds__(64 * WAVES) void w4a16_stream_kernel(const __half* __restrict__ A, BW4Params p,
                                                                   __half* __restrict__ C, int M, int N, int K,
                                                                   int splits, float* __restrict__ partial,
                                                                   unsigned* __restrict__ counters) {
-    __shared__ float red[WAVES - 1][16][64];
+    constexpr int CT = 16 * CPL;   // columns per block
+    typedef uint32_t wvec __attribute__((ext_vector_type(CPL)));
+    __shared__ float red[WAVES - 1][4 * CPL][64];
     __shared__ int last_flag;
     const int ntile = blockIdx.x, split = blockIdx.y;
-    const int n0 = ntile * kStreamCols;
+    const int n0 = ntile * CT;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, kb = lane >> 4;
-    const int ncol = n0 + 4 * r;
-    const int ncolc = ncol < N ? ncol : N - 4;          // clamped: loads stay in bounds, results unused
+    const int ncol = n0 + CPL * r;
+    const int ncolc = ncol < N ? ncol : N - CPL;        // clamped: loads stay in bounds, results unused
     const int KP = K / 8;
     const int ksteps = (KP + 3) / 4;   // 32 k per MFMA step
     const int ks0 = (int)((long)ksteps * split / splits), ks1 = (int)((long)ksteps * (split + 1) / splits);
     const long zrow = N / 8;
 
-    v4f acc[MT][4];
+    v4f acc[MT][CPL];
 #pragma unroll
     for (int mt = 0; mt < MT; mt++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[mt][j] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < CPL; j++) acc[mt][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
     for (int ks = ks0 + wave; ks < ks1; ks += WAVES * U) {
-        uint4 w[U];
-        uint2 s[U];
+        wvec w[U];
+        uint32_t s[U][CPL / 2];
         uint32_t z[U];
         uint4 a[U][MT];
         bool kv[U];
@@ -89,8 +92,10 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_stream_kernel(const __half* 
             kv[u] = (ks + WAVES * u < ks1) && (kp < KP);
             const int kpc = kv[u] ? kp : 0;
             const int g = kpc * 8 / p.group;
-            w[u] = ld_nt(reinterpret_cast<const uint4*>(p.W + (long)kpc * N + ncolc));
-            s[u] = *reinterpret_cast<const uint2*>(p.scales + (long)g * N + ncolc);
+            w[u] = __builtin_nontemporal_load(reinterpret_cast<const wvec*>(p.W + (long)kpc * N + ncolc));
+            const uint32_t* sp = reinterpret_cast<const uint32_t*>(p.scales + (long)g * N + ncolc);
+#pragma unroll
+            for (int h = 0; h < CPL / 2; h++) s[u][h] = sp[h];
             z[u] = p.qzeros ? p.qzeros[(long)g * zrow + ncolc / 8] : 0x88888888u;
 #pragma unroll
             for (int mt = 0; mt < MT; mt++) {
@@ -101,14 +106,13 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_stream_kernel(const __half* 
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const uint32_t zs = (z[u] >> (4 * (ncolc & 7))) & 0xFFFFu;
-            const uint32_t wv[4] = {w[u].x, w[u].y, w[u].z, w[u].w};
-            const uint32_t sv[4] = {s[u].x & 0xFFFFu, s[u].x >> 16, s[u].y & 0xFFFFu, s[u].y >> 16};
-            v8h wf[4];
+            const uint32_t zs = z[u] >> (4 * (ncolc & 7));
+            v8h wf[CPL];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < CPL; j++) {
                 const uint32_t zj = 0x6400u + ((zs >> (4 * j)) & 0xFu);
-                const uint4 d = dequant8(wv[j], zj | (zj << 16), sv[j] | (sv[j] << 16));
+                const uint32_t sj = (j & 1) ? s[u][j / 2] >> 16 : s[u][j / 2] & 0xFFFFu;
+                const uint4 d = dequant8(w[u][j], zj | (zj << 16), sj | (sj << 16));
                 __builtin_memcpy(&wf[j], &d, 16);
             }
 #pragma unroll
@@ -119,7 +123,7 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_stream_kernel(const __half* 
                 v8h af;
                 __builtin_memcpy(&af, &av, 16);
 #pragma unroll
-                for (int j = 0; j < 4; j++)
+                for (int j = 0; j < CPL; j++)
                     acc[mt][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wf[j], af, acc[mt][j], 0, 0, 0);
             }
         }
@@ -130,14 +134,14 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_stream_kernel(const __half* 
     for (int mt = 0; mt < MT; mt++) {
         if (wave > 0) {
 #pragma unroll
-            for (int j = 0; j < 4; j++)
+            for (int j = 0; j < CPL; j++)
 #pragma unroll
                 for (int i = 0; i < 4; i++) red[wave - 1][j * 4 + i][lane] = acc[mt][j][i];
         }
         __syncthreads();
         if (wave == 0) {
 #pragma unroll
-            for (int j = 0; j < 4; j++)
+            for (int j = 0; j < CPL; j++)
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     float v = acc[mt][j][i];
@@ -148,7 +152,7 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_stream_kernel(const __half* 
         }
         __syncthreads();
     }
-    // acc[mt][j][i] of lane l is C[m = 16 mt + l%16][n = n0 + 16 (l/16) + 4 i + j]
+    // acc[mt][j][i] of lane l is C[m = 16 mt + l%16][n = n0 + CPL (4 (l/16) + i) + j]
     if (splits <= 1) {
         if (wave != 0) return;
 #pragma unroll
@@ -157,13 +161,15 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_stream_kernel(const __half* 
             if (m >= M) continue;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int n = n0 + 16 * kb + 4 * i;
+                const int n = n0 + CPL * (4 * kb + i);
                 if (n >= N) continue;
-                const uint32_t lo = (uint32_t)__half_as_ushort(__float2half_rn(acc[mt][0][i])) |
-                                    ((uint32_t)__half_as_ushort(__float2half_rn(acc[mt][1][i])) << 16);
-                const uint32_t hi = (uint32_t)__half_as_ushort(__float2half_rn(acc[mt][2][i])) |
-                                    ((uint32_t)__half_as_ushort(__float2half_rn(acc[mt][3][i])) << 16);
-                *reinterpret_cast<uint2*>(C + (long)m * N + n) = make_uint2(lo, hi);
+                uint32_t pk[CPL / 2];
+#pragma unroll
+                for (int h = 0; h < CPL / 2; h++)
+                    pk[h] = (uint32_t)__half_as_ushort(__float2half_rn(acc[mt][2 * h][i])) |
+                            ((uint32_t)__half_as_ushort(__float2half_rn(acc[mt][2 * h + 1][i])) << 16);
+#pragma unroll
+                for (int h = 0; h < CPL / 2; h++) reinterpret_cast<uint32_t*>(C + (long)m * N + n)[h] = pk[h];
             }
         }
         return;
@@ -176,10 +182,10 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_stream_kernel(const __half* 
             if (m >= M) continue;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int n = n0 + 16 * kb + 4 * i;
+                const int n = n0 + CPL * (4 * kb + i);
                 if (n >= N) continue;
 #pragma unroll
-                for (int j = 0; j < 4; j++) publish(mine + (long)m * N + n + j, acc[mt][j][i]);
+                for (int j = 0; j < CPL; j++) publish(mine + (long)m * N + n + j, acc[mt][j][i]);
             }
         }
         drain();
@@ -188,10 +194,10 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_stream_kernel(const __half* 
     if (threadIdx.x == 0) last_flag = (take_ticket(&counters[ntile]) == (unsigned)splits - 1);
     __syncthreads();
     if (!last_flag) return;
-    // the last block sums the 64-column x min(M, 16 MT)-row tile, 4 columns per thread
+    // the last block sums the CT-column x min(M, 16 MT)-row tile, 4 columns per thread
     const int rows = M < 16 * MT ? M : 16 * MT;
-    for (int e = threadIdx.x; e < rows * (kStreamCols / 4); e += 64 * WAVES) {
-        const int m = e / (kStreamCols / 4), n = n0 + 4 * (e % (kStreamCols / 4));
+    for (int e = threadIdx.x; e < rows * (CT / 4); e += 64 * WAVES) {
+        const int m = e / (CT / 4), n = n0 + 4 * (e % (CT / 4));
         if (n >= N) continue;
         // splits summed in order, four splits' loads in flight at a time
         float v[4] = {0.f, 0.f, 0.f, 0.f};
@@ -428,17 +434,19 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
         const int per_split = (ksteps + splits - 1) / splits;
         const int want = (per_split + unroll - 1) / unroll;
         const int waves = want >= 16 && mt <= 2 ? 16 : (want >= 8 ? 8 : 4);
-#define LT_W4S(MT_, U_, W_)                                                                                       \
-    hipLaunchKernelGGL((w4a16_stream_kernel<MT_, U_, W_>), grid, dim3(64 * W_), 0, s, a, p, cc, m, n, k, splits, \
-                       partial, counters)
+#define LT_W4S(MT_, U_, W_, CPL_)                                                                          \
+    hipLaunchKernelGGL((w4a16_stream_kernel<MT_, U_, W_, CPL_>), grid, dim3(64 * W_), 0, s, a, p, cc, m, n, k, \
+                       splits, partial, counters)
+        // (two columns per lane, 32-column tiles, measured slower at M = 1:
+        // half-width loads leave fewer bytes in flight than the extra CUs add)
         if (mt <= 1) {
-            if (waves == 16) LT_W4S(1, 8, 16); else if (waves == 8) LT_W4S(1, 8, 8); else LT_W4S(1, 8, 4);
+            if (waves == 16) LT_W4S(1, 8, 16, 4); else if (waves == 8) LT_W4S(1, 8, 8, 4); else LT_W4S(1, 8, 4, 4);
         } else if (mt <= 2) {
-            if (waves == 16) LT_W4S(2, 4, 16); else if (waves == 8) LT_W4S(2, 4, 8); else LT_W4S(2, 4, 4);
+            if (waves == 16) LT_W4S(2, 4, 16, 4); else if (waves == 8) LT_W4S(2, 4, 8, 4); else LT_W4S(2, 4, 4, 4);
         } else if (mt <= 4) {
-            if (waves == 8) LT_W4S(4, 2, 8); else LT_W4S(4, 2, 4);
+            if (waves == 8) LT_W4S(4, 2, 8, 4); else LT_W4S(4, 2, 4, 4);
         } else {
-            if (waves == 8) LT_W4S(8, 2, 8); else LT_W4S(8, 2, 4);
+            if (waves == 8) LT_W4S(8, 2, 8, 4); else LT_W4S(8, 2, 4, 4);
         }
 #undef LT_W4S
         LT_TRY_LAUNCH("w4a16 stream launch");
