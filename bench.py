@@ -9,7 +9,10 @@ dgemm 4096^3, hgemm 4096^3, w4a16 M in {1,16,128,2048}).  `cpu_baseline`
 times the oracle's sgemv on the host (one thread).  `measured_peaks` holds
 what this device delivers (MFMA probe loops, a read-only HBM stream and a
 copy); every routine carries its fraction of the datasheet peak and
-`frac_measured`, its fraction of the measured one.  `vendor_reference` times
+`frac_measured`, its fraction of the measured one, and `max_rel_err_vs_fp64`,
+its largest error against an FP64 evaluation of the same inputs (divided by
+the sum of absolute terms, |A||B| or |A||x|; reduction: by the sum).
+`vendor_reference` times
 torch.matmul (hipBLASLt / rocBLAS) on the GEMM shapes, for context only.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
@@ -96,7 +99,59 @@ def cpu_baseline(seconds=10.0):
             break
     gbs = SG_BYTES * reps / el / 1e9
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"full 16384x8192 sgemv, {reps} runs in {el:.1f} s (oracle/oracle.c orc_sgemv, FP64 dot per row)"}
+            "sample": f"full 16384x8192 sgemv, {reps} runs in {el:.1f} s (oracle/oracle.c orc_sgemv, FP64 dot per row)",
+            "per_routine": cpu_oracle_routines()}
+
+
+def cpu_oracle_routines(budget_s=2.0):
+    """The oracle's other checks timed on the host on bounded samples of each
+    configuration (threads as stated), so each routine's CPU path has a rate."""
+    import numpy as np
+    import _oracle as orc
+    from libthom_amd import inputs
+    threads = orc.THREADS
+
+    def rate(fn):
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            fn()
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= budget_s:
+                return reps / el
+
+    out = {}
+    x = inputs.uniform_f32(1_000_000, 0.0, 1.0, 1234, inputs.S_REDUCE)
+    r = rate(lambda: orc.sum_f64(x))
+    out["reduction_n1000000"] = {"GB/s": round(4e6 * r / 1e9, 3), "cores": 1, "sample": "full n = 1e6, FP64 sum"}
+    y = inputs.uniform_f32(1_000_000, -1.0, 1.0, 1234, inputs.S_SDOT_Y)
+    r = rate(lambda: orc.dot_f64(x, y))
+    out["sdot_n1000000"] = {"GB/s": round(8e6 * r / 1e9, 3), "cores": 1, "sample": "full n = 1e6, FP64 dot"}
+    n = 512
+    A = inputs.normal_f64(n * n, 1234, inputs.S_DGEMM_A).reshape(n, n)
+    B = inputs.normal_f64(n * n, 1234, inputs.S_DGEMM_B).reshape(n, n)
+    r = rate(lambda: orc.dgemm(A, B, threads=threads))
+    out["dgemm"] = {"GFLOP/s": round(2 * n ** 3 * r / 1e9, 2), "cores": threads,
+                    "sample": "512^3 of the 4096^3 configuration (N(0,1)), blocked FP64"}
+    r = rate(lambda: orc.ozaki_dgemm(A, B, 8, threads=threads))
+    out["ozaki8"] = {"GFLOP/s_fp64_equiv": round(2 * n ** 3 * r / 1e9, 2), "cores": threads,
+                     "sample": "512^3, 8 slices (split, exact level sums, FP64 recombination)"}
+    n = 1024
+    Ah = inputs.uniform_f16(n * n, -1, 1, 1234, 31).reshape(n, n)
+    Bh = inputs.uniform_f16(n * n, -1, 1, 1234, 32).reshape(n, n)
+    r = rate(lambda: orc.hgemm(Ah, Bh, threads=threads))
+    out["hgemm"] = {"GFLOP/s": round(2 * n ** 3 * r / 1e9, 2), "cores": threads,
+                    "sample": "1024^3, FP32 accumulation of FP16 products"}
+    Aw, q, sc, z = inputs.w4a16_problem(16, 11008, 4096, 128)
+    packed, qz = inputs.pack_w4(q), inputs.pack_zeros(z)
+    Wdq = orc.w4_dequant(packed, sc, qz, 128)
+    r = rate(lambda: orc.w4a16_rows(Aw, Wdq, threads=threads))
+    out["w4a16_M16"] = {"GFLOP/s": round(2 * 16 * 11008 * 4096 * r / 1e9, 2), "cores": threads,
+                        "sample": "M = 16 configuration on the dequantised weights (FP32 k-ascending)"}
+    r = rate(lambda: orc.w4_dequant(packed, sc, qz, 128))
+    out["w4_dequant"] = {"weights/s": round(4096 * 11008 * r, 0), "cores": 1,
+                         "sample": "4096 x 11008 int4 weights -> FP16"}
+    return out
 
 
 def bench_routines(torch, R, timer, dev, quick):
@@ -118,8 +173,10 @@ def bench_routines(torch, R, timer, dev, quick):
         R.reduce_sum(x, o)
         ms = timer.per_launch_ms(lambda: R.reduce_sum(x, o), it, flush if n < 100_000_000 else None)
         gbs = 4 * n / ms / 1e6
+        ref = x.double().sum().item()
         out[f"reduction_n{n}"] = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
-                                  "GFLOP/s": round((n - 1) / ms / 1e6, 1)}
+                                  "GFLOP/s": round((n - 1) / ms / 1e6, 1),
+                                  "max_rel_err_vs_fp64": abs(o.item() - ref) / abs(ref)}
         del x
     # sdot
     for n in (1_000_000, 250_000_000):
@@ -131,8 +188,11 @@ def bench_routines(torch, R, timer, dev, quick):
         R.sdot(xs, ys, o)
         ms = timer.per_launch_ms(lambda: R.sdot(xs, ys, o), it, flush if n < 100_000_000 else None)
         gbs = 8 * n / ms / 1e6
-        out[f"sdot_n{n}"] = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
-        del xs, ys
+        xd, yd = xs.double(), ys.double()
+        err = abs(o.item() - torch.dot(xd, yd).item()) / torch.dot(xd.abs(), yd.abs()).item()
+        out[f"sdot_n{n}"] = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+                             "max_rel_err_vs_fp64": err}
+        del xs, ys, xd, yd
     torch.cuda.empty_cache()
     # dgemm: native FP64 MFMA vs split-precision on FP8 MFMA, errors against
     # the compensated (Dot2) product, relative to (|A||B|)_ij, on both inputs
@@ -182,9 +242,12 @@ def bench_routines(torch, R, timer, dev, quick):
         R.hgemm(A, B, C)
         ms = timer.per_launch_ms(lambda: R.hgemm(A, B, C), max(3, it // 4))
         tf = 2 * n ** 3 / ms / 1e9
+        rows = torch.arange(0, n, n // 128, device=dev)
+        Ar, Bd = A[rows].double(), B.double()
+        err = ((C[rows].double() - Ar @ Bd).abs() / (Ar.abs() @ Bd.abs())).max().item()
         out[f"hgemm_{n}"] = {"ms": round(ms, 4), "TFLOP/s": round(tf, 1), "frac_fp16_mfma": round(tf / FP16_PEAK, 4),
-                             "config": repr(R.hgemm_default_config(n, n, n))}
-        del A, B, C
+                             "config": repr(R.hgemm_default_config(n, n, n)), "max_rel_err_vs_fp64": err}
+        del A, B, C, Ar, Bd
     # batched hgemm: 256 uneven problems, one persistent launch and its hipGraph
     from libthom_amd import tuner
     dims = R.batched_problem_dims()
@@ -218,10 +281,15 @@ def bench_routines(torch, R, timer, dev, quick):
         ms = timer.per_launch_ms(lambda: R.w4a16(At, Wt, St, Zt, G, Ct), it, flush)
         byts = 2 * M * Kw + Kw // 8 * Nw * 4 + Kw // G * Nw * 2 + Kw // G * Nw // 2 + 2 * M * Nw
         flops = 2 * M * Nw * Kw
+        rows = torch.arange(0, M, max(1, M // 64), device=dev)
+        Wd = R.w4_dequant(Wt, St, Zt, G).double()   # bit-exact dequantisation
+        Ar = At[rows].double()
+        err = ((Ct[rows].double() - Ar @ Wd).abs() / (Ar.abs() @ Wd.abs())).max().item()
         out[f"w4a16_M{M}"] = {"us": round(ms * 1e3, 2), "GB/s": round(byts / ms / 1e6, 1),
                               "frac_hbm": round(byts / ms / 1e6 / HBM_PEAK_GBS, 4),
                               "TFLOP/s": round(flops / ms / 1e9, 2),
-                              "frac_fp16_mfma": round(flops / ms / 1e9 / FP16_PEAK, 4)}
+                              "frac_fp16_mfma": round(flops / ms / 1e9 / FP16_PEAK, 4), "max_rel_err_vs_fp64": err}
+        del Wd, Ar
     return out
 
 
@@ -314,6 +382,10 @@ def main():
     ms = start.elapsed_time(end) / args.steps
     gbs = SG_BYTES / ms / 1e6
     log(f"sgemv 16384x8192: {ms * 1e3:.1f} us/step, {gbs:.0f} GB/s")
+    Ad, xd = A.double(), x.double()
+    sg_err = ((y.double() - Ad @ xd).abs() / (Ad.abs() @ xd.abs())).max().item()
+    del Ad, xd
+    torch.cuda.empty_cache()
 
     if args.copy_probe:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)
@@ -357,6 +429,7 @@ def main():
         "dtype": "f32", "data": "synthetic (SplitMix64 seed 1234, U[-1,1))",
         "config": {"workload": "sgemv row-major y = A x, A 16384 x 8192 FP32", "M": M_SG, "N": N_SG,
                    "bytes_per_step": SG_BYTES},
+        "max_rel_err_vs_fp64": sg_err,
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("sgemv_16384x8192")},
         "cpu_baseline": cpu,
