@@ -22,6 +22,9 @@
 // zero-points reach LDS by DMA; W is dequantised in registers, one uint32 per
 // MFMA fragment).  Needs group % 64 == 0, N % 32 == 0 and 16-byte aligned
 // scales/zero-points; the automatic choice falls back to kind 2 otherwise.
+//
+// kind 4, decode (M <= 4): v_dot2 on exactly dequantised (q - z) pairs, the
+// group scale applied once per group; no split-K (described at the kernel).
 #include "w4_core2.hpp"
 
 namespace lt {
@@ -226,6 +229,237 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_stream_kernel(const __half* 
     if (threadIdx.x == 0) reset_counter(&counters[ntile]);
 }
 
+// ---- kind 4: decode (M <= 4) on the vector ALUs -------------------------
+// At M <= 4 the MFMA path spends its time dequantising (a 16x16 MFMA tile
+// wastes 15/16 of its rows) and the weight stream waits on the VALU.  Here
+// one packed uint32 (8 k of one column) costs 1 shift, 4 v_and_or_b32, 4
+// packed fp16 subtractions and 4 v_dot2_f32_f16 per activation row:
+//   (x & 0x000F000F) | 0x64006400 = fp16 pair (1024 + q_k, 1024 + q_k+4)
+//   (x & 0x00F000F0) | 0x64006400 = (1024 + 16 q_k+1, 1024 + 16 q_k+5)
+// and the same on x >> 8 for k+2/k+6, k+3/k+7.  Subtracting 1024 + z (resp.
+// 1024 + 16 z) leaves (q - z), resp. 16 (q - z), exactly; v_dot2 multiplies
+// them with the activation pairs (exact products, FP32 sums) and the group's
+// scale is applied once per group and column: acc += s * (De + Do / 16).
+// The scaled weight is never rounded to fp16, so the result is closer to the
+// exact product than the dequantise-then-multiply order; it stays within the
+// product tolerance of w4a16.cpp:194-197 and is checked against the oracle.
+//
+// Layout: RG weight rows per wave load, 64 / RG lanes per row, 4 columns per
+// lane (one uint4): a block covers 256 / RG columns and all of K.  The
+// block's A rows (M <= MR) sit in LDS in the (k, k + 4) pair order.  Waves
+// take the K / group groups round robin, QG = group / (8 RG) row quads each,
+// in batches of UQ quads; the next batch's loads are issued before the
+// current one is consumed.  Weight, scale and zero-point loads are buffer
+// loads with the nontemporal hint (each byte is used once).  The RG row
+// groups of a wave are summed by lane exchange, the waves in LDS in wave
+// order: the result is deterministic.
+typedef _Float16 h2v_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u4v_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u2v_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ h2v_t as_h2(uint32_t v) {
+    h2v_t h;
+    __builtin_memcpy(&h, &v, 4);
+    return h;
+}
+// v_and_or_b32 with both constants in registers (one VOP3 instruction; the
+// compiler otherwise splits it into v_and + v_or with literal operands)
+__device__ __forceinline__ uint32_t and_or(uint32_t x, uint32_t m, uint32_t c) {
+    uint32_t r;
+    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(m), "v"(c));
+    return r;
+}
+
+constexpr int kDecodeWaves = 8;
+constexpr int kDecodeThreads = 64 * kDecodeWaves;
+constexpr int kDecodeAStage = 8;   // A elements (uint4) staged per thread: MR * K / 8 <= 4096
+constexpr int kDecodeMaxMK = kDecodeAStage * kDecodeThreads * 8;
+
+template <int MR, int RG, int QG, int UQ>
+__global__ __launch_bounds__(kDecodeThreads) void w4a16_decode_kernel(const __half* __restrict__ A, BW4Params p,
+                                                                      __half* __restrict__ C, int M, int N, int K) {
+    static_assert(UQ % QG == 0, "a batch holds whole groups");
+    constexpr int WAVES = kDecodeWaves, NT = kDecodeThreads, ASTG = kDecodeAStage;
+    constexpr int LPR = 64 / RG, CT = 4 * LPR, GB = UQ / QG;
+    constexpr int AUX = 2;   // nontemporal
+    extern __shared__ uint4 dsm[];
+    const int KP = K / 8;
+    const int NG = KP / (RG * QG);
+    uint4* As = dsm;                                         // [MR][KP], (k, k + 4) pair order
+    float* red = reinterpret_cast<float*>(dsm + MR * KP);    // [WAVES][MR][CT]
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int cl = lane % LPR, rl = lane / LPR;
+    const int n0 = blockIdx.x * CT;
+    const int ncol = n0 + 4 * cl;
+    const int colc = ncol < N ? ncol : N - 4;
+    const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)p.W, 0, KP * N * 4, 0x00020000);
+    const __amdgpu_buffer_rsrc_t sr = __builtin_amdgcn_make_buffer_rsrc((void*)p.scales, 0, NG * N * 2, 0x00020000);
+    const __amdgpu_buffer_rsrc_t zr =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(p.qzeros ? p.qzeros : p.W), 0, NG * (N / 8) * 4, 0x00020000);
+    const bool has_z = p.qzeros != nullptr;
+    const int wv_off = (rl * N + colc) * 4, sv_off = colc * 2, zv_off = (colc / 8) * 4;
+    const int zsh = 4 * (colc & 7);
+    // A loads first: the first weight batch is issued before their LDS writes wait
+    uint4 av[ASTG];
+    bool aval[ASTG];
+#pragma unroll
+    for (int i = 0; i < ASTG; i++) {
+        const int e = threadIdx.x + i * NT;
+        const int m = e / KP, kp = e % KP;
+        aval[i] = e < MR * KP && m < M;
+        av[i] = *reinterpret_cast<const uint4*>(aval[i] ? A + (long)m * K + 8 * kp : A);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const uint32_t MASK_LO = 0x000F000Fu, MASK_HI = 0x00F000F0u, MAGIC = 0x64006400u;
+    float acc[MR][4];
+#pragma unroll
+    for (int m = 0; m < MR; m++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[m][j] = 0.f;
+    // this wave's groups: wave, wave + WAVES, ... (ngw of them), in batches of GB
+    const int ngw = wave < NG ? (NG - 1 - wave) / WAVES + 1 : 0;
+    const int nb = (ngw + GB - 1) / GB;
+    const int glast = wave + WAVES * (ngw - 1);
+    struct Batch {
+        u4v_t w[UQ];
+        u2v_t s[GB];
+        uint32_t z[GB];
+    };
+    auto load = [&](Batch& bt, int b) {
+#pragma unroll
+        for (int gi = 0; gi < GB; gi++) {
+            int g = wave + WAVES * (b * GB + gi);
+            g = g <= glast ? g : glast;   // past the wave's last group: a repeat, not used
+            bt.s[gi] = __builtin_amdgcn_raw_buffer_load_b64(sr, sv_off, g * N * 2, AUX);
+            bt.z[gi] = has_z ? __builtin_amdgcn_raw_buffer_load_b32(zr, zv_off, g * (N / 8) * 4, AUX) : 0x88888888u;
+#pragma unroll
+            for (int qi = 0; qi < QG; qi++)
+                bt.w[gi * QG + qi] = __builtin_amdgcn_raw_buffer_load_b128(wr, wv_off, (g * QG + qi) * RG * N * 4, AUX);
+        }
+    };
+    auto compute = [&](const Batch& bt, int b) {
+#pragma unroll
+        for (int gi = 0; gi < GB; gi++) {
+            const int g = wave + WAVES * (b * GB + gi);
+            if (g > glast) break;
+            const uint32_t zs = bt.z[gi] >> zsh;
+            uint32_t zze[4], zzo[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t zj = (zs >> (4 * j)) & 0xFu;
+                const uint32_t e = 0x6400u | zj, o = 0x6400u | (zj << 4);
+                zze[j] = e | (e << 16);
+                zzo[j] = o | (o << 16);
+            }
+            float de[MR][4], dd[MR][4];
+#pragma unroll
+            for (int m = 0; m < MR; m++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) de[m][j] = dd[m][j] = 0.f;
+#pragma unroll
+            for (int qi = 0; qi < QG; qi++) {
+                const int row = (g * QG + qi) * RG + rl;
+                uint4 a[MR];
+#pragma unroll
+                for (int m = 0; m < MR; m++) a[m] = As[m * KP + row];
+                const u4v_t wq = bt.w[gi * QG + qi];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t x = wq[j];
+                    const uint32_t t = x >> 8;
+                    const h2v_t e0 = as_h2(and_or(x, MASK_LO, MAGIC)) - as_h2(zze[j]);
+                    const h2v_t e1 = as_h2(and_or(x, MASK_HI, MAGIC)) - as_h2(zzo[j]);
+                    const h2v_t e2 = as_h2(and_or(t, MASK_LO, MAGIC)) - as_h2(zze[j]);
+                    const h2v_t e3 = as_h2(and_or(t, MASK_HI, MAGIC)) - as_h2(zzo[j]);
+#pragma unroll
+                    for (int m = 0; m < MR; m++) {
+                        de[m][j] = __builtin_amdgcn_fdot2(e0, as_h2(a[m].x), de[m][j], false);
+                        dd[m][j] = __builtin_amdgcn_fdot2(e1, as_h2(a[m].y), dd[m][j], false);
+                        de[m][j] = __builtin_amdgcn_fdot2(e2, as_h2(a[m].z), de[m][j], false);
+                        dd[m][j] = __builtin_amdgcn_fdot2(e3, as_h2(a[m].w), dd[m][j], false);
+                    }
+                }
+            }
+            const uint32_t sv[2] = {bt.s[gi].x, bt.s[gi].y};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float sj =
+                    __half2float(__ushort_as_half((unsigned short)((j & 1) ? sv[j / 2] >> 16 : sv[j / 2] & 0xFFFFu)));
+#pragma unroll
+                for (int m = 0; m < MR; m++) acc[m][j] = fmaf(sj, fmaf(dd[m][j], 0.0625f, de[m][j]), acc[m][j]);
+            }
+        }
+    };
+    Batch cur;
+    if (nb > 0) load(cur, 0);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < ASTG; i++) {
+        const int e = threadIdx.x + i * NT;
+        if (e < MR * KP) As[e] = aval[i] ? perm_k8(av[i]) : make_uint4(0, 0, 0, 0);
+    }
+    // LDS writes visible; the weight loads stay in flight (no vmcnt wait)
+    asm volatile("s_waitcnt lgkmcnt(0)\n s_barrier" ::: "memory");
+    for (int b = 0; b + 1 < nb; b++) {
+        Batch nxt;
+        load(nxt, b + 1);
+        compute(cur, b);
+        cur = nxt;
+    }
+    if (nb > 0) compute(cur, nb - 1);
+#pragma unroll
+    for (int off = LPR; off < 64; off <<= 1)
+#pragma unroll
+        for (int m = 0; m < MR; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[m][j] += __shfl_xor(acc[m][j], off);
+    if (rl == 0) {
+#pragma unroll
+        for (int m = 0; m < MR; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) red[(wave * MR + m) * CT + 4 * cl + j] = acc[m][j];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < MR * CT; t += NT) {
+        const int m = t / CT, c = t % CT;
+        float v = 0.f;
+        for (int w2 = 0; w2 < WAVES; w2++) v += red[(w2 * MR + m) * CT + c];
+        if (m < M && n0 + c < N) C[(long)m * N + n0 + c] = __float2half_rn(v);
+    }
+}
+
+typedef void (*decode_fn)(const __half*, BW4Params, __half*, int, int, int);
+struct DecodeEntry {
+    int mr, rg, qg;
+    decode_fn fn;
+};
+#define LT_W4D(mr, rg, qg) {mr, rg, qg, w4a16_decode_kernel<mr, rg, qg, (qg > 4 ? qg : 4)>},
+#define LT_W4D_QG(mr, rg) LT_W4D(mr, rg, 1) LT_W4D(mr, rg, 2) LT_W4D(mr, rg, 4) LT_W4D(mr, rg, 8)
+const DecodeEntry kDecode[] = {LT_W4D_QG(1, 4) LT_W4D_QG(1, 8) LT_W4D_QG(2, 4) LT_W4D_QG(2, 8) LT_W4D_QG(4, 4)
+                                   LT_W4D_QG(4, 8)};
+#undef LT_W4D_QG
+#undef LT_W4D
+
+// Kind-4 launch shape for M x N x K (group): rows per wave load RG = 256 /
+// tile_n, MR = 1, 2 or 4 >= M, QG = group / (8 RG).  Null when the shape is
+// outside the kernel's domain.
+const DecodeEntry* decode_entry(long M, long N, long K, int group, int tile_n) {
+    if (tile_n != 32 && tile_n != 64) return nullptr;
+    const int rg = 256 / tile_n;
+    const int mr = M <= 1 ? 1 : (M <= 2 ? 2 : 4);
+    if (M < 1 || M > 4 || K <= 0 || N < 4 || group % (8 * rg) != 0) return nullptr;
+    if ((long)mr * K > kDecodeMaxMK) return nullptr;          // A staging bound (LDS <= 64 KiB)
+    if ((K / 8) * N * 4 >= (1L << 31)) return nullptr;        // 32-bit buffer offsets
+    const int qg = group / (8 * rg);
+    for (const DecodeEntry& e : kDecode)
+        if (e.mr == mr && e.rg == rg && e.qg == qg) return &e;
+    return nullptr;
+}
+size_t decode_lds(const DecodeEntry& e, long K) {
+    return (size_t)e.mr * (size_t)(K / 8) * 16 + (size_t)kDecodeWaves * e.mr * (4 * (64 / e.rg)) * 4;
+}
+
 template <int BM, int BN, int BK>
 __global__ __launch_bounds__(256) void w4a16_tiled_kernel(const __half* __restrict__ A, BW4Params p,
                                                           __half* __restrict__ C, int M, int N, int K, int tiles_m,
@@ -339,7 +573,16 @@ int tiled2_tail(const Tiled2Entry& e, long M, long N, long K, int* full) {
 void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_w4a16_config* out) {
     lthip_w4a16_config c = cfg ? *cfg : lthip_w4a16_config{0, 0, 0, 0};
     if (c.kind == 0) {
-        if (M <= 64) {
+        if (M <= 2 || (M <= 4 && N < 10240)) {
+            // decode on the vector ALUs; 64 columns per block (RG = 4) once
+            // N / 64 fills most CUs, 32 otherwise.  At M = 3, 4 and wide N
+            // the MFMA stream kernel is as fast (measured).  The launch falls
+            // back to kind 1 where the kernel's conditions do not hold.
+            c.kind = 4;
+            c.tile_n = N >= 10240 ? 64 : 32;
+            c.tile_m = 0;
+            c.split_k = 1;
+        } else if (M <= 64) {
             // no split-K: the launch runs up to 16 waves per column tile, one
             // unrolled round of weight loads each, and the FP32 partials of a
             // split cost more than the extra blocks gain
@@ -406,6 +649,35 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
     BW4Params p{W_packed, reinterpret_cast<const __half*>(scales), qzeros, group};
     const int m = (int)M, n = (int)N, k = (int)K;
     hipStream_t s = as_stream(stream);
+    if (c.kind == 4 && K == 0) c.kind = 1;   // nothing to accumulate: the stream kernel writes the zeros
+    if (c.kind == 4) {
+        const bool explicit4 = cfg && cfg->kind == 4;
+        const DecodeEntry* e = decode_entry((long)M, (long)N, (long)K, group, c.tile_n);
+        if (explicit4 && c.split_k != 1) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: kind 4 takes split_k = 1");
+        if (e) {
+            const size_t lds = decode_lds(*e, (long)K);
+            if (lds > 65536) {   // above the default dynamic LDS limit
+                const int st = check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(e->fn),
+                                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                                         "w4a16 decode attribute");
+                if (st != LTHIP_OK) return st;
+            }
+            const int ct = 4 * (64 / e->rg);
+            hipLaunchKernelGGL(e->fn, dim3((unsigned)((n + ct - 1) / ct)), dim3(kDecodeThreads), lds, s,
+                               reinterpret_cast<const __half*>(A), p, reinterpret_cast<__half*>(C), m, n, k);
+            LT_TRY_LAUNCH("w4a16 decode launch");
+            return LTHIP_OK;
+        }
+        if (explicit4)
+            return fail(LTHIP_ERR_UNSUPPORTED,
+                        "w4a16: kind 4 needs M <= 4, K > 0, tile_n 32 or 64, group %% (2048 / tile_n) == 0, "
+                        "M * K <= %d and K * N / 2 < 2^31",
+                        kDecodeMaxMK);
+        c.kind = 1;   // automatic choice outside the decode kernel's domain
+        c.split_k = 1;
+        c.tile_n = kStreamCols;
+        c.tile_m = 16;
+    }
     if (c.kind == 1) {
         if (M > 128) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: kind 1 supports M <= 128");
         const int ntiles = (n + kStreamCols - 1) / kStreamCols;

@@ -162,3 +162,83 @@ def test_w4a16_pipelined_rejects_unsupported(dev):
     with pytest.raises(LthipError):
         R.w4a16(t(A, dev), i32(inputs.pack_w4(q), dev), t(s, dev), i32(inputs.pack_zeros(z), dev), 128,
                 config=W4A16Config(3, 128, 128, 1))
+
+
+KIND4 = [W4A16Config(4, 64, 0, 1), W4A16Config(4, 32, 0, 1)]
+
+
+@pytest.mark.parametrize("cfg", KIND4)
+@pytest.mark.parametrize("M", [1, 2, 3, 4])
+@pytest.mark.parametrize("group", [32, 64, 128, 256])
+def test_w4a16_decode(dev, orc, cfg, M, group):
+    from libthom_amd._lib import LthipError
+    N, K = 520, 1024
+    A, q, s, z = inputs.w4a16_problem(M, N, K, group, seed=131 + M)
+    packed, qz = inputs.pack_w4(q), inputs.pack_zeros(z)
+    args = (t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), group)
+    if group % (2048 // cfg.tile_n) != 0:
+        with pytest.raises(LthipError):
+            R.w4a16(*args, config=cfg)
+        return
+    C = R.w4a16(*args, config=cfg).cpu().numpy()
+    w4_close(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, group)))
+
+
+@pytest.mark.parametrize("cfg", KIND4)
+@pytest.mark.parametrize("N,K", [(8, 128), (24, 256), (4104, 384), (264, 32768)])
+def test_w4a16_decode_edges(dev, orc, cfg, N, K):
+    # narrower than one block, ragged last block, K at the A-staging bound
+    # (dynamic LDS above 64 KiB)
+    M = 1
+    A, q, s, z = inputs.w4a16_problem(M, N, K, 128, seed=N + K)
+    packed, qz = inputs.pack_w4(q), inputs.pack_zeros(z)
+    C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), 128, config=cfg).cpu().numpy()
+    w4_close(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, 128)))
+
+
+def test_w4a16_decode_exact_and_symmetric(dev, orc):
+    # scale a power of two, A small integers: every product and FP32 sum is
+    # exact, so the decode kernel matches the oracle bit for bit
+    N, K = 200, 1024
+    rng = np.random.default_rng(21)
+    q = rng.integers(0, 16, (K, N)).astype(np.uint8)
+    z = rng.integers(0, 16, (K // 128, N)).astype(np.uint8)
+    packed, qz = orc.w4_pack(q), orc.w4_pack_zeros(z)
+    for M in (1, 2, 3, 4):
+        A = rng.integers(-3, 4, (M, K)).astype(np.float16)
+        for scale in (1.0, 0.5):
+            s = np.full((K // 128, N), scale, np.float16)
+            for cfg in KIND4:
+                C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), 128, config=cfg).cpu().numpy()
+                assert np.array_equal(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, 128))), (M, scale, cfg)
+                C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), None, 128, config=cfg).cpu().numpy()
+                assert np.array_equal(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, None, 128))), (M, scale, cfg)
+
+
+def test_w4a16_decode_auto_matches_stream(dev, orc):
+    # the automatic choice at M <= 2 is the decode kernel; M = 3, 4 at wide N
+    # stay on the MFMA stream kernel; both within the product tolerance
+    for M, N in ((1, 11008), (2, 4096), (3, 4096), (4, 10240)):
+        K = 1024
+        A, q, s, z = inputs.w4a16_problem(M, N, K, 128, seed=7 * M)
+        packed, qz = inputs.pack_w4(q), inputs.pack_zeros(z)
+        args = (t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), 128)
+        ref = orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, 128))
+        w4_close(R.w4a16(*args).cpu().numpy(), ref)
+        w4_close(R.w4a16(*args, config=W4A16Config(1, 64, 16, 1)).cpu().numpy(), ref)
+
+
+def test_w4a16_decode_rejects_unsupported(dev):
+    from libthom_amd._lib import LthipError
+    A, q, s, z = inputs.w4a16_problem(5, 256, 512, 128, seed=9)
+    packed, qz = i32(inputs.pack_w4(q), dev), i32(inputs.pack_zeros(z), dev)
+    At, st = t(A, dev), t(s, dev)
+    for M, cfg in ((5, W4A16Config(4, 64, 0, 1)),      # M > 4
+                   (1, W4A16Config(4, 48, 0, 1)),      # tile_n not built
+                   (1, W4A16Config(4, 64, 0, 2))):     # split-K
+        with pytest.raises(LthipError):
+            R.w4a16(At[:M], packed, st, qz, 128, config=cfg)
+    A4, q4, s4, z4 = inputs.w4a16_problem(4, 64, 16384, 128, seed=10)   # M * K above the staging bound
+    with pytest.raises(LthipError):
+        R.w4a16(t(A4, dev), i32(inputs.pack_w4(q4), dev), t(s4, dev), i32(inputs.pack_zeros(z4), dev), 128,
+                config=W4A16Config(4, 64, 0, 1))
