@@ -195,10 +195,13 @@ typedef struct lthip_w4a16_config {
     int kind;        /* 0 = auto, 1 = small-M split-K stream, 2 = tiled MFMA, 3 = pipelined tiled MFMA
                         (DMA ring; group % 64 == 0, N % 32 == 0, 16-byte aligned scales/zero-points),
                         4 = decode on the vector ALUs (M <= 4, group % (2048 / tile_n) == 0,
-                        M * K <= 32768, K * N / 2 < 2^31; no workspace) */
-    int tile_n;      /* kind 1: 64 columns per block (fixed); kinds 2, 3: BN; kind 4: 64 or 32 */
-    int tile_m;      /* kinds 2, 3: BM; kind 4: ignored (1, 2 or 4 rows from M) */
-    int split_k;     /* K splits (>= 1); > 1 needs the workspace; kind 4: 1 */
+                        M * K <= 32768, K * N / 2 < 2^31; no workspace),
+                        5 = skinny-M MFMA (M <= 64, N >= 8, group % 32 == 0, group <= 256,
+                        K * N / 2 < 2^31; no workspace) */
+    int tile_n;      /* kind 1: 64 columns per block (fixed); kinds 2, 3: BN; kind 4: 64 or 32;
+                        kind 5: 64 */
+    int tile_m;      /* kinds 2, 3: BM; kinds 4, 5: ignored (row tiles from M) */
+    int split_k;     /* K splits (>= 1); > 1 needs the workspace; kinds 4, 5: 1 */
 } lthip_w4a16_config;
 
 size_t lthip_w4a16_workspace_size(size_t M, size_t N, size_t K, const lthip_w4a16_config *cfg);

@@ -25,6 +25,9 @@
 //
 // kind 4, decode (M <= 4): v_dot2 on exactly dequantised (q - z) pairs, the
 // group scale applied once per group; no split-K (described at the kernel).
+//
+// kind 5, skinny M (<= 64): the kind-1 lane mapping on MFMA with kind 4's
+// exact (q - z) operands and per-group scaling; no split-K.
 #include "w4_core2.hpp"
 
 namespace lt {
@@ -460,6 +463,206 @@ size_t decode_lds(const DecodeEntry& e, long K) {
     return (size_t)e.mr * (size_t)(K / 8) * 16 + (size_t)kDecodeWaves * e.mr * (4 * (64 / e.rg)) * 4;
 }
 
+// ---- kind 5: skinny M (<= 64) on MFMA with exact (q - z) operands --------
+// The lane mapping of the kind-1 stream kernel (lane l loads columns
+// n0 + 4(l%16)..+3 of weight row 4 ks + l/16; MFMA 16x16x32 j takes columns
+// n0 + 4 r + j as its A operand, 16 activation rows as its B operand), with
+// the decode kernel's arithmetic: each uint32 becomes four fp16 pairs of
+// exact (q - z) (3 shifts, 4 v_and_or_b32, 4 packed subtractions), the MFMAs
+// accumulate one group into fresh FP32 accumulators, and the group's scales
+// (16 consecutive columns per lane, the output layout's) fold them into the
+// totals: acc += s * group_acc.  A block covers 64 columns and all of K; its
+// waves take the groups round robin, batches of UQ k-steps with the next
+// batch's loads in flight; weights, scales and zero-points are nontemporal
+// buffer loads.  Waves are summed in LDS in wave order (deterministic).
+// Output lane l: C[16 mt + l%16][n0 + 16 (l/16) + 4 i + j] = acc[mt][j][i].
+template <int MT, int QS, int WAVES, int UQ>
+__global__ __launch_bounds__(64 * WAVES) void w4a16_skinny_kernel(const __half* __restrict__ A, BW4Params p,
+                                                                  __half* __restrict__ C, int M, int N, int K) {
+    static_assert(UQ % QS == 0, "a batch holds whole groups");
+    constexpr int GB = UQ / QS, AUX = 2;
+    extern __shared__ float sred[];   // [WAVES][MT][4 i][4 j][64]
+    const int KP = K / 8;
+    const int NG = KP / (4 * QS);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane & 15, kb = lane >> 4;
+    const int n0 = blockIdx.x * 64;
+    const int colw = n0 + 4 * r < N ? n0 + 4 * r : N - 4;
+    // scales of columns n0 + 16 kb .. +15 as two 8-column chunks; a chunk that
+    // starts at or past N is never stored, so clamping it keeps loads in range
+    int cs0 = n0 + 16 * kb, cs1 = cs0 + 8;
+    cs0 = cs0 < N - 8 ? cs0 : N - 8;
+    cs1 = cs1 < N - 8 ? cs1 : N - 8;
+    const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)p.W, 0, KP * N * 4, 0x00020000);
+    const __amdgpu_buffer_rsrc_t sr = __builtin_amdgcn_make_buffer_rsrc((void*)p.scales, 0, NG * N * 2, 0x00020000);
+    const __amdgpu_buffer_rsrc_t zr =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(p.qzeros ? p.qzeros : p.W), 0, NG * (N / 8) * 4, 0x00020000);
+    const __amdgpu_buffer_rsrc_t ar = __builtin_amdgcn_make_buffer_rsrc((void*)A, 0, M * K * 2, 0x00020000);
+    const bool has_z = p.qzeros != nullptr;
+    const int wv_off = (kb * N + colw) * 4, zv_off = (colw / 8) * 4;
+    const int zsh = 4 * (colw & 7);
+    int av_off[MT];
+    bool mv[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+        const int m = 16 * mt + r;
+        mv[mt] = m < M;
+        av_off[mt] = ((mv[mt] ? m : M - 1) * K + 8 * kb) * 2;
+    }
+    const uint32_t MASK = 0x000F000Fu, MAGIC = 0x64006400u;
+    v4f acc[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[mt][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    const int ngw = wave < NG ? (NG - 1 - wave) / WAVES + 1 : 0;
+    const int nb = (ngw + GB - 1) / GB;
+    const int glast = wave + WAVES * (ngw - 1);
+    struct Batch {
+        u4v_t w[UQ];
+        u4v_t a[UQ][MT];
+        u4v_t s0[GB], s1[GB];
+        uint32_t z[GB];
+    };
+    auto load = [&](Batch& bt, int b) {
+#pragma unroll
+        for (int gi = 0; gi < GB; gi++) {
+            int g = wave + WAVES * (b * GB + gi);
+            g = g <= glast ? g : glast;   // past the wave's last group: a repeat, not used
+            bt.z[gi] = has_z ? __builtin_amdgcn_raw_buffer_load_b32(zr, zv_off, g * (N / 8) * 4, AUX) : 0x88888888u;
+#pragma unroll
+            for (int qi = 0; qi < QS; qi++) {
+                const int ks = g * QS + qi;
+                bt.w[gi * QS + qi] = __builtin_amdgcn_raw_buffer_load_b128(wr, wv_off, ks * 4 * N * 4, AUX);
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++)
+                    bt.a[gi * QS + qi][mt] = __builtin_amdgcn_raw_buffer_load_b128(ar, av_off[mt], ks * 64, 0);
+            }
+            bt.s0[gi] = __builtin_amdgcn_raw_buffer_load_b128(sr, cs0 * 2, g * N * 2, AUX);
+            bt.s1[gi] = __builtin_amdgcn_raw_buffer_load_b128(sr, cs1 * 2, g * N * 2, AUX);
+        }
+    };
+    auto compute = [&](const Batch& bt, int b) {
+#pragma unroll
+        for (int gi = 0; gi < GB; gi++) {
+            const int g = wave + WAVES * (b * GB + gi);
+            if (g > glast) break;
+            const uint32_t zs = bt.z[gi] >> zsh;
+            uint32_t zz[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t e = 0x6400u | ((zs >> (4 * j)) & 0xFu);
+                zz[j] = e | (e << 16);
+            }
+            v4f ag[MT][4];
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) ag[mt][j] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int qi = 0; qi < QS; qi++) {
+                v8h af[MT];
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++) {
+                    const u4v_t a = bt.a[gi * QS + qi][mt];
+                    uint4 pa = perm_k8(make_uint4(a.x, a.y, a.z, a.w));
+                    if (!mv[mt]) pa = make_uint4(0, 0, 0, 0);
+                    __builtin_memcpy(&af[mt], &pa, 16);
+                }
+                const u4v_t wq = bt.w[gi * QS + qi];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t x = wq[j];
+                    // fp16 pairs (k, k + 4) of exact (q - z), in the perm_k8 order
+                    const h2v_t e0 = as_h2(and_or(x, MASK, MAGIC)) - as_h2(zz[j]);
+                    const h2v_t e1 = as_h2(and_or(x >> 4, MASK, MAGIC)) - as_h2(zz[j]);
+                    const h2v_t e2 = as_h2(and_or(x >> 8, MASK, MAGIC)) - as_h2(zz[j]);
+                    const h2v_t e3 = as_h2(and_or(x >> 12, MASK, MAGIC)) - as_h2(zz[j]);
+                    v8h wf;
+                    wf[0] = e0[0];
+                    wf[1] = e0[1];
+                    wf[2] = e1[0];
+                    wf[3] = e1[1];
+                    wf[4] = e2[0];
+                    wf[5] = e2[1];
+                    wf[6] = e3[0];
+                    wf[7] = e3[1];
+#pragma unroll
+                    for (int mt = 0; mt < MT; mt++)
+                        ag[mt][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wf, af[mt], ag[mt][j], 0, 0, 0);
+                }
+            }
+            const uint32_t sv[8] = {bt.s0[gi].x, bt.s0[gi].y, bt.s0[gi].z, bt.s0[gi].w,
+                                    bt.s1[gi].x, bt.s1[gi].y, bt.s1[gi].z, bt.s1[gi].w};
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int c = 4 * i + j;   // the scale of column n0 + 16 kb + c
+                    const float sc = __half2float(
+                        __ushort_as_half((unsigned short)((c & 1) ? sv[c / 2] >> 16 : sv[c / 2] & 0xFFFFu)));
+#pragma unroll
+                    for (int mt = 0; mt < MT; mt++) acc[mt][j][i] = fmaf(sc, ag[mt][j][i], acc[mt][j][i]);
+                }
+        }
+    };
+    Batch cur;
+    if (nb > 0) load(cur, 0);
+    for (int b = 0; b + 1 < nb; b++) {
+        Batch nxt;
+        load(nxt, b + 1);
+        compute(cur, b);
+        cur = nxt;
+    }
+    if (nb > 0) compute(cur, nb - 1);
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) sred[(((wave * MT + mt) * 4 + i) * 4 + j) * 64 + lane] = acc[mt][j][i];
+    __syncthreads();
+    // thread (mt, i, lane): four consecutive columns, waves summed in order
+    for (int t = threadIdx.x; t < MT * 4 * 64; t += 64 * WAVES) {
+        const int l = t & 63, i = (t >> 6) & 3, mt = t >> 8;
+        const int m = 16 * mt + (l & 15), n = n0 + 16 * (l >> 4) + 4 * i;
+        float v[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int w2 = 0; w2 < WAVES; w2++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) v[j] += sred[(((w2 * MT + mt) * 4 + i) * 4 + j) * 64 + l];
+        if (m < M && n < N) {
+            const uint32_t lo = (uint32_t)__half_as_ushort(__float2half_rn(v[0])) |
+                                ((uint32_t)__half_as_ushort(__float2half_rn(v[1])) << 16);
+            const uint32_t hi = (uint32_t)__half_as_ushort(__float2half_rn(v[2])) |
+                                ((uint32_t)__half_as_ushort(__float2half_rn(v[3])) << 16);
+            *reinterpret_cast<uint2*>(C + (long)m * N + n) = make_uint2(lo, hi);
+        }
+    }
+}
+
+struct SkinnyEntry {
+    int mt, qs, waves;
+    decode_fn fn;
+};
+#define LT_W4K(mt, qs, w) {mt, qs, w, w4a16_skinny_kernel<mt, qs, w, (qs > 4 ? qs : 4)>},
+#define LT_W4K_QS(mt, w) LT_W4K(mt, 1, w) LT_W4K(mt, 2, w) LT_W4K(mt, 4, w) LT_W4K(mt, 8, w)
+// one row tile: 16 waves where the batch fits 128 VGPRs (group 128), else 8
+const SkinnyEntry kSkinny[] = {LT_W4K(1, 1, 8) LT_W4K(1, 2, 8) LT_W4K(1, 4, 16) LT_W4K(1, 8, 8) LT_W4K_QS(2, 4)
+                                   LT_W4K_QS(4, 4)};
+#undef LT_W4K_QS
+#undef LT_W4K
+
+// Kind-5 launch entry for M x N x K (group), or null outside its domain.
+const SkinnyEntry* skinny_entry(long M, long N, long K, int group) {
+    if (M < 1 || M > 64 || K <= 0 || N < 8 || group % 32 != 0) return nullptr;
+    if ((K / 8) * N * 4 >= (1L << 31) || M * K * 2 >= (1L << 31)) return nullptr;   // 32-bit buffer offsets
+    const int mt = M <= 16 ? 1 : (M <= 32 ? 2 : 4), qs = group / 32;
+    for (const SkinnyEntry& e : kSkinny)
+        if (e.mt == mt && e.qs == qs) return &e;
+    return nullptr;
+}
+
 template <int BM, int BN, int BK>
 __global__ __launch_bounds__(256) void w4a16_tiled_kernel(const __half* __restrict__ A, BW4Params p,
                                                           __half* __restrict__ C, int M, int N, int K, int tiles_m,
@@ -576,20 +779,17 @@ void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_
         if (M <= 2 || (M <= 4 && N < 10240)) {
             // decode on the vector ALUs; 64 columns per block (RG = 4) once
             // N / 64 fills most CUs, 32 otherwise.  At M = 3, 4 and wide N
-            // the MFMA stream kernel is as fast (measured).  The launch falls
-            // back to kind 1 where the kernel's conditions do not hold.
+            // the skinny MFMA kernel is faster (measured).  The launch falls
+            // back to kind 5, then kind 1, where the conditions do not hold.
             c.kind = 4;
             c.tile_n = N >= 10240 ? 64 : 32;
             c.tile_m = 0;
             c.split_k = 1;
         } else if (M <= 64) {
-            // no split-K: the launch runs up to 16 waves per column tile, one
-            // unrolled round of weight loads each, and the FP32 partials of a
-            // split cost more than the extra blocks gain
-            c.kind = 1;
+            c.kind = 5;
+            c.tile_n = 64;
+            c.tile_m = 0;
             c.split_k = 1;
-            c.tile_n = kStreamCols;
-            c.tile_m = 16;
         } else if (N % 32 == 0) {
             // pipelined tiles: the largest tile once M fills it; the launch
             // shares the last partial round of tiles out by splitting K
@@ -649,7 +849,7 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
     BW4Params p{W_packed, reinterpret_cast<const __half*>(scales), qzeros, group};
     const int m = (int)M, n = (int)N, k = (int)K;
     hipStream_t s = as_stream(stream);
-    if (c.kind == 4 && K == 0) c.kind = 1;   // nothing to accumulate: the stream kernel writes the zeros
+    if ((c.kind == 4 || c.kind == 5) && K == 0) c.kind = 1;   // nothing to accumulate: the stream kernel writes zeros
     if (c.kind == 4) {
         const bool explicit4 = cfg && cfg->kind == 4;
         const DecodeEntry* e = decode_entry((long)M, (long)N, (long)K, group, c.tile_n);
@@ -673,7 +873,27 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
                         "w4a16: kind 4 needs M <= 4, K > 0, tile_n 32 or 64, group %% (2048 / tile_n) == 0, "
                         "M * K <= %d and K * N / 2 < 2^31",
                         kDecodeMaxMK);
-        c.kind = 1;   // automatic choice outside the decode kernel's domain
+        c.kind = 5;   // automatic choice outside the decode kernel's domain
+        c.split_k = 1;
+        c.tile_n = 64;
+        c.tile_m = 0;
+    }
+    if (c.kind == 5) {
+        const bool explicit5 = cfg && cfg->kind == 5;
+        if (explicit5 && (c.split_k != 1 || c.tile_n != 64))
+            return fail(LTHIP_ERR_INVALID_ARG, "w4a16: kind 5 takes tile_n = 64 and split_k = 1");
+        const SkinnyEntry* e = skinny_entry((long)M, (long)N, (long)K, group);
+        if (e) {
+            const size_t lds = (size_t)e->waves * e->mt * 16 * 64 * 4;
+            hipLaunchKernelGGL(e->fn, dim3((unsigned)((n + 63) / 64)), dim3((unsigned)(64 * e->waves)), lds, s,
+                               reinterpret_cast<const __half*>(A), p, reinterpret_cast<__half*>(C), m, n, k);
+            LT_TRY_LAUNCH("w4a16 skinny launch");
+            return LTHIP_OK;
+        }
+        if (explicit5 && K > 0)
+            return fail(LTHIP_ERR_UNSUPPORTED,
+                        "w4a16: kind 5 needs M <= 64, N >= 8, group %% 32 == 0, group <= 256 and 32-bit offsets");
+        c.kind = 1;   // outside the skinny kernel's domain (or K == 0): the stream kernel
         c.split_k = 1;
         c.tile_n = kStreamCols;
         c.tile_m = 16;

@@ -242,3 +242,55 @@ def test_w4a16_decode_rejects_unsupported(dev):
     with pytest.raises(LthipError):
         R.w4a16(t(A4, dev), i32(inputs.pack_w4(q4), dev), t(s4, dev), i32(inputs.pack_zeros(z4), dev), 128,
                 config=W4A16Config(4, 64, 0, 1))
+
+
+KIND5 = W4A16Config(5, 64, 0, 1)
+
+
+@pytest.mark.parametrize("M", [1, 5, 16, 17, 33, 64])
+@pytest.mark.parametrize("group", [32, 64, 128, 256])
+def test_w4a16_skinny(dev, orc, M, group):
+    N, K = 520, 1024
+    A, q, s, z = inputs.w4a16_problem(M, N, K, group, seed=171 + M)
+    packed, qz = inputs.pack_w4(q), inputs.pack_zeros(z)
+    C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), group, config=KIND5).cpu().numpy()
+    w4_close(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, group)))
+
+
+@pytest.mark.parametrize("N,K", [(8, 128), (24, 256), (4104, 384)])
+def test_w4a16_skinny_edges(dev, orc, N, K):
+    M = 20
+    A, q, s, z = inputs.w4a16_problem(M, N, K, 128, seed=N + 3 * K)
+    packed, qz = inputs.pack_w4(q), inputs.pack_zeros(z)
+    C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), 128, config=KIND5).cpu().numpy()
+    w4_close(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, 128)))
+
+
+def test_w4a16_skinny_exact_and_symmetric(dev, orc):
+    N, K = 200, 1024
+    rng = np.random.default_rng(23)
+    q = rng.integers(0, 16, (K, N)).astype(np.uint8)
+    z = rng.integers(0, 16, (K // 64, N)).astype(np.uint8)
+    packed, qz = orc.w4_pack(q), orc.w4_pack_zeros(z)
+    for M in (3, 16, 40):
+        A = rng.integers(-3, 4, (M, K)).astype(np.float16)
+        for scale in (1.0, 0.25):
+            s = np.full((K // 64, N), scale, np.float16)
+            C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), 64, config=KIND5).cpu().numpy()
+            assert np.array_equal(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, 64))), (M, scale)
+            C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), None, 64, config=KIND5).cpu().numpy()
+            assert np.array_equal(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, None, 64))), (M, scale)
+
+
+def test_w4a16_skinny_rejects_unsupported(dev):
+    from libthom_amd._lib import LthipError
+    A, q, s, z = inputs.w4a16_problem(65, 256, 512, 128, seed=19)
+    packed, qz = i32(inputs.pack_w4(q), dev), i32(inputs.pack_zeros(z), dev)
+    At, st = t(A, dev), t(s, dev)
+    for M, cfg in ((65, KIND5), (16, W4A16Config(5, 32, 0, 1)), (16, W4A16Config(5, 64, 0, 2))):
+        with pytest.raises(LthipError):
+            R.w4a16(At[:M], packed, st, qz, 128, config=cfg)
+    A2, q2, s2, z2 = inputs.w4a16_problem(8, 64, 512, 16, seed=20)   # group 16
+    with pytest.raises(LthipError):
+        R.w4a16(t(A2, dev), i32(inputs.pack_w4(q2), dev), t(s2, dev), i32(inputs.pack_zeros(z2), dev), 16,
+                config=KIND5)
