@@ -54,16 +54,25 @@ W4A16_SPACE: List[Tuple[int, int, int, int]] = (
     [(2, 64, 64, s) for s in (1, 2, 4, 8)] + [(2, 128, 64, s) for s in (1, 2, 4, 8)] +
     [(2, 128, 128, s) for s in (1, 2, 4)] +
     [(2, bn, bm, 1) for bm, bn in ((128, 256), (256, 128), (256, 256))] +
-    [(3, 128, 128, 1), (3, 128, 128, 2), (3, 256, 128, 1), (3, 128, 256, 1), (3, 256, 256, 1)])
+    [(3, 128, 128, 1), (3, 128, 128, 2), (3, 256, 128, 1), (3, 128, 256, 1), (3, 256, 256, 1)] +
+    [(4, 64, 0, 1), (4, 32, 0, 1), (5, 64, 0, 1)])
 
 
 def valid(cfg: Tuple[int, int, int, int], M: int, K: int, N: Optional[int] = None,
           group: Optional[int] = None) -> bool:
     """Whether the library runs cfg on the shape; kind 3 needs N % 32 == 0 and
-    group % 64 == 0 (false when N or group is not given)."""
-    kind, _, _, split = cfg
+    group % 64 == 0, kind 4 M <= 4, group % (2048 / tile_n) == 0 and M K <=
+    32768, kind 5 M <= 64 and group % 32 == 0 (kinds 3-5: false when N or
+    group is not given)."""
+    kind, tile_n, _, split = cfg
     if kind == 1:
         return M <= 128 and split <= max(1, K // 32)
+    if kind == 4:
+        return N is not None and group is not None and M <= 4 and group % (2048 // tile_n) == 0 and \
+            M * K <= 32768 and (K // 8) * N * 4 < 2 ** 31 and split == 1
+    if kind == 5:
+        return N is not None and group is not None and M <= 64 and N >= 8 and group % 32 == 0 and \
+            group <= 256 and (K // 8) * N * 4 < 2 ** 31 and split == 1
     if kind == 3:
         return N is not None and group is not None and N % 32 == 0 and group % 64 == 0 and \
             split <= max(1, K // 256)
