@@ -3,9 +3,12 @@
 // (uniform nibbles, FP16 scales in [0.005, 0.02], per-(group, column) int4
 // zero-points); a trailing "sym" token instead quantizes fp16 weights with the
 // reference's symmetric quantizer (quantize.cpp:8-53, zero-point 8).
-// Implementations: "hip" (size-based choice), "stream" (decode kernel),
-// "tiled" (MFMA tiles, dequantized into LDS), "pipelined" (DMA-ring MFMA tiles,
-// dequantized in registers), "tuned" (tuning database, else "hip").
+// Implementations: "hip" (size-based choice), "stream" (split-K MFMA stream
+// kernel), "decode" (vector-ALU kernel, M <= 4), "skinny" (MFMA with exact
+// (q - z) operands, M <= 64), "tiled" (MFMA tiles, dequantized into LDS),
+// "pipelined" (DMA-ring MFMA tiles, dequantized in registers), "tuned"
+// (tuning database, else "hip").  Shapes outside an implementation's domain
+// run the size-based choice.
 // `tune w4a16 M N K` sweeps every configuration into the database.
 #include <algorithm>
 #include <cmath>
@@ -51,6 +54,16 @@ static float w4a16_stream(const Problem& p) {
     if (p.M > 128) return run_cfg(p, nullptr);
     return run_cfg(p, &c);
 }
+static float w4a16_decode(const Problem& p) {
+    if (p.M > 4) return run_cfg(p, nullptr);
+    lthip_w4a16_config c{4, p.N >= 10240 ? 64 : 32, 0, 1};
+    return run_cfg(p, &c);
+}
+static float w4a16_skinny(const Problem& p) {
+    if (p.M > 64) return run_cfg(p, nullptr);
+    lthip_w4a16_config c{5, 64, 0, 1};
+    return run_cfg(p, &c);
+}
 static float w4a16_tiled(const Problem& p) {
     lthip_w4a16_config c{2, 128, 128, 1};
     return run_cfg(p, &c);
@@ -69,8 +82,8 @@ static float w4a16_tuned(const Problem& p) {
 
 static const std::unordered_map<std::string, W4A16Function> w4a16Impls = {
 #if LIB_HAS(W4A16_HIP)
-    {"hip", w4a16_hip},   {"stream", w4a16_stream}, {"tiled", w4a16_tiled}, {"pipelined", w4a16_pipelined},
-    {"tuned", w4a16_tuned},
+    {"hip", w4a16_hip},       {"stream", w4a16_stream}, {"decode", w4a16_decode}, {"skinny", w4a16_skinny},
+    {"tiled", w4a16_tiled},   {"pipelined", w4a16_pipelined}, {"tuned", w4a16_tuned},
 #endif
 };
 
@@ -300,6 +313,11 @@ static void tune(int argc, char* argv[]) {
     computeReference(M, N, K);
     const Problem p{gpuA.p, gpuW.p, gpuScales.p, gpuZ.p, gpuC.p, (size_t)M, (size_t)N, (size_t)K};
     std::vector<lthip_w4a16_config> cands;
+    if (M <= 4) {
+        cands.push_back({4, 64, 0, 1});
+        cands.push_back({4, 32, 0, 1});
+    }
+    if (M <= 64) cands.push_back({5, 64, 0, 1});
     if (M <= 128)
         for (int s : {1, 2, 3, 4, 6, 8, 12, 16}) cands.push_back({1, 64, 16, s});
     for (auto bm_bn : {std::pair<int, int>{64, 64}, {64, 128}, {128, 128}, {128, 256}, {256, 128}, {256, 256}})

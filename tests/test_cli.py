@@ -58,6 +58,10 @@ def test_missing_arguments_list_implementations():
     ("w4a16", "hip", 16, 1024, 1024, 2),
     ("w4a16", "tiled", 192, 512, 1024, 2),
     ("w4a16", "stream", 1, 11008, 4096, 2, 1, "sym"),
+    ("w4a16", "decode", 1, 11008, 4096, 2),
+    ("w4a16", "decode", 3, 4096, 1024, 2, 1, "sym"),
+    ("w4a16", "skinny", 16, 11008, 4096, 2),
+    ("w4a16", "skinny", 40, 520, 1024, 2, 1, "sym"),
     ("w4a16", "pipelined", 300, 512, 1024, 2),
     ("w4a16", "pipelined", 2048, 11008, 4096, 2, 1, "sym"),
 ])
