@@ -212,6 +212,20 @@ int lthip_w4a16_gemm(const uint16_t *A, const uint32_t *W_packed, const uint16_t
  * check of dequantize_w4 (quantize.cpp:58-65) on the GPU. */
 int lthip_w4_dequant(const uint32_t *W_packed, const uint16_t *scales, const uint32_t *qzeros,
                      uint16_t *W, size_t K, size_t N, int group, void *stream);
+/* The quantizer on the GPU: W (K x N fp16, row-major, 16-byte aligned) ->
+ * W_packed ((K/8) x N, pack_w4 of quantize.cpp:42-53), scales ((K/group) x N
+ * fp16) and, for mode 1, qzeros ((K/group) x (N/8) uint32, column n in bits
+ * 4*(n%8), z stored as-is).  mode 0 replaces quantize_w4 (quantize.cpp:8-37):
+ * scale = fp16(2 max|W| / 15) (1 for an all-zero group), q = clamp(
+ * nearbyint(W / scale) + 8, 0, 15); qzeros is not written and may be null.
+ * mode 1 is the asymmetric min/max variant (no counterpart in the
+ * reference): lo = min(0, min W), hi = max(0, max W), scale = fp16((hi - lo)
+ * / 15) (1 when hi == lo), z = clamp(nearbyint(-lo / scale), 0, 15), q =
+ * clamp(nearbyint(W / scale) + z, 0, 15).  IEEE FP32 divisions, ties to even:
+ * bit-identical to the host quantizer for finite W.  K % group == 0,
+ * group % 8 == 0, group <= 1024, N % 8 == 0. */
+int lthip_w4_quantize(const uint16_t *W, uint32_t *W_packed, uint16_t *scales, uint32_t *qzeros,
+                      size_t K, size_t N, int group, int mode, void *stream);
 
 /* ---- learned tuner: the cost-model MLP (no counterpart in the reference; its
  *      tuner is the exhaustive search of hgemm.cu:258-323).  FP32, row-major:

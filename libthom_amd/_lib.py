@@ -100,6 +100,7 @@ _SIGS = {
     "lthip_w4a16_gemm": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_size, c_size, c_int,
                                  ctypes.POINTER(W4A16Config), c_vp, c_size, c_vp]),
     "lthip_w4_dequant": (c_int, [c_vp, c_vp, c_vp, c_vp, c_size, c_size, c_int, c_vp]),
+    "lthip_w4_quantize": (c_int, [c_vp, c_vp, c_vp, c_vp, c_size, c_size, c_int, c_int, c_vp]),
 }
 
 _lock = threading.Lock()

@@ -238,6 +238,21 @@ def w4_dequant(W_packed: torch.Tensor, scales: torch.Tensor, qzeros: Optional[to
     return out
 
 
+def w4_quantize(W: torch.Tensor, group: int = 128, asymmetric: bool = False):
+    """quantize_w4 + pack_w4 (quantize.cpp:8-53) on the GPU: W (K, N) fp16 ->
+    (W_packed (K/8, N) int32, scales (K/group, N) fp16, qzeros (K/group, N/8)
+    int32 or None).  asymmetric: the min/max variant with zero-points."""
+    _need_cuda(W)
+    K, N = W.shape
+    assert W.dtype == torch.float16 and W.is_contiguous()
+    packed = torch.empty(K // 8, N, dtype=torch.int32, device=W.device)
+    scales = torch.empty(K // group, N, dtype=torch.float16, device=W.device)
+    qzeros = torch.empty(K // group, N // 8, dtype=torch.int32, device=W.device) if asymmetric else None
+    call("lthip_w4_quantize", W.data_ptr(), packed.data_ptr(), scales.data_ptr(), _ptr(qzeros), K, N, group,
+         1 if asymmetric else 0, _stream(W.device))
+    return packed, scales, qzeros
+
+
 class GroupedHgemm:
     """A plan for many independent hgemms in one persistent launch.
 

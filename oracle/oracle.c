@@ -405,6 +405,37 @@ void orc_w4_quantize_sym(const uint16_t *W, uint8_t *q, uint16_t *scales, int K,
     free(mx);
 }
 
+void orc_w4_quantize_asym(const uint16_t *W, uint8_t *q, uint8_t *z, uint16_t *scales, int K, int N, int group) {
+    int groups = K / group;
+    float *lo = (float *)malloc(sizeof(float) * (size_t)N);
+    float *hi = (float *)malloc(sizeof(float) * (size_t)N);
+    for (int g = 0; g < groups; g++) {
+        for (int n = 0; n < N; n++) lo[n] = hi[n] = 0.0f;     /* the range always holds 0 */
+        for (int k = g * group; k < (g + 1) * group; k++)
+            for (int n = 0; n < N; n++) {
+                float w = orc_f16_to_f32(W[(size_t)k * N + n]);
+                lo[n] = fminf(lo[n], w);
+                hi[n] = fmaxf(hi[n], w);
+            }
+        for (int n = 0; n < N; n++) {
+            float r = hi[n] - lo[n];
+            uint16_t sb = orc_f32_to_f16(r > 0.0f ? r / 15.0f : 1.0f);
+            scales[(size_t)g * N + n] = sb;
+            float s = orc_f16_to_f32(sb);
+            int zz = (int)nearbyint(-lo[n] / s);
+            z[(size_t)g * N + n] = (uint8_t)(zz < 0 ? 0 : (zz > 15 ? 15 : zz));
+        }
+        for (int k = g * group; k < (g + 1) * group; k++)
+            for (int n = 0; n < N; n++) {
+                float s = orc_f16_to_f32(scales[(size_t)g * N + n]);
+                int v = (int)nearbyint(orc_f16_to_f32(W[(size_t)k * N + n]) / s) + z[(size_t)g * N + n];
+                q[(size_t)k * N + n] = (uint8_t)(v < 0 ? 0 : (v > 15 ? 15 : v));
+            }
+    }
+    free(lo);
+    free(hi);
+}
+
 void orc_w4_pack(const uint8_t *q, uint32_t *packed, int K, int N) {
     for (int pr = 0; pr < K / 8; pr++)                            /* quantize.cpp:43-52 */
         for (int n = 0; n < N; n++) {

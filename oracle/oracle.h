@@ -92,6 +92,14 @@ void orc_hgemm(const uint16_t *A, const uint16_t *B, uint16_t *C, int M, int N, 
 /* ---- w4a16 ------------------------------------------------------------ */
 /* quantize.cpp:8-37 (symmetric, ZERO_POINT 8, GROUP_SIZE = group) */
 void orc_w4_quantize_sym(const uint16_t *W, uint8_t *q, uint16_t *scales, int K, int N, int group);
+/* Asymmetric min/max variant of quantize.cpp:8-37 (no counterpart in the
+ * reference; the zero-point layout is GPTQ's qzeros, z stored as-is).  Per
+ * (group, column): lo = min(0, min W), hi = max(0, max W) in FP32;
+ * scale = fp16((hi - lo) / 15), or 1 when hi == lo; z = clamp(nearbyint(-lo /
+ * scale), 0, 15); q = clamp(nearbyint(W / scale) + z, 0, 15).  Every division
+ * is one IEEE FP32 division and nearbyint rounds half to even, as in
+ * quantize.cpp:31-33.  z: (K/group) x N bytes. */
+void orc_w4_quantize_asym(const uint16_t *W, uint8_t *q, uint8_t *z, uint16_t *scales, int K, int N, int group);
 /* quantize.cpp:42-53: packed[k/8][n] holds row k in bits 4*(k%8) */
 void orc_w4_pack(const uint8_t *q, uint32_t *packed, int K, int N);
 void orc_w4_unpack(const uint32_t *packed, uint8_t *q, int K, int N);

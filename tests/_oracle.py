@@ -46,6 +46,7 @@ _sigs = {
     "orc_ozaki_dgemm": (None, [P, P, P, I, I, I, D, D, I, I]),
     "orc_hgemm": (None, [P, P, P, I, I, I, ctypes.c_uint16, ctypes.c_uint16, I]),
     "orc_w4_quantize_sym": (None, [P, P, P, I, I, I]),
+    "orc_w4_quantize_asym": (None, [P, P, P, P, I, I, I]),
     "orc_w4_pack": (None, [P, P, I, I]),
     "orc_w4_unpack": (None, [P, P, I, I]),
     "orc_w4_pack_zeros": (None, [P, P, I, I]),
@@ -189,6 +190,16 @@ def w4_quantize_sym(W, group=128):
     s = np.empty((K // group, N), dtype=np.float16)
     _L.orc_w4_quantize_sym(_p(W.view(np.uint16)), _p(q), _p(s.view(np.uint16)), K, N, group)
     return q, s
+
+
+def w4_quantize_asym(W, group=128):
+    W = np.ascontiguousarray(W, dtype=np.float16)
+    K, N = W.shape
+    q = np.empty((K, N), dtype=np.uint8)
+    z = np.empty((K // group, N), dtype=np.uint8)
+    s = np.empty((K // group, N), dtype=np.float16)
+    _L.orc_w4_quantize_asym(_p(W.view(np.uint16)), _p(q), _p(z), _p(s.view(np.uint16)), K, N, group)
+    return q, z, s
 
 
 def w4_pack(q):

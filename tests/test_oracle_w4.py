@@ -110,3 +110,49 @@ def test_w4a16_rows_small_integers_exact(orc):
     ref = (A.astype(np.int64) @ (q.astype(np.int64) - 8)).astype(np.float16)
     assert np.array_equal(C, ref)
     assert np.array_equal(orc.w4a16_rows(A, Wd, 1, 2), ref[1:3])
+
+
+def test_quantize_asym_exact_round_trip(orc):
+    # W = (q - z) s with s a power of two and each (group, column) holding both
+    # q = 0 and q = 15: the quantizer must give back q, z and s exactly
+    K, N, group = 256, 24, 64
+    rng = np.random.default_rng(31)
+    q = rng.integers(0, 16, (K, N)).astype(np.uint8)
+    for g in range(K // group):
+        q[g * group, :] = 0
+        q[g * group + 1, :] = 15
+    z = rng.integers(0, 16, (K // group, N)).astype(np.uint8)
+    s = (2.0 ** rng.integers(-12, 2, (K // group, N))).astype(np.float16)
+    gi = np.repeat(np.arange(K // group), group)
+    W = ((q.astype(np.float32) - z[gi]) * s[gi].astype(np.float32)).astype(np.float16)
+    q2, z2, s2 = orc.w4_quantize_asym(W, group)
+    assert np.array_equal(s2, s) and np.array_equal(z2, z) and np.array_equal(q2, q)
+
+
+def test_quantize_asym_properties(orc):
+    K, N, group = 256, 40, 128
+    rng = np.random.default_rng(32)
+    W = rng.uniform(-1, 1, (K, N)).astype(np.float16)
+    W[:, 3] = np.abs(W[:, 3])                               # all positive: z = 0
+    W[:, 4] = -np.abs(W[:, 4])                              # all negative: z = 15
+    W[:128, 5] = 0                                          # an all-zero group: s = 1, z = 0, q = 0
+    q, z, s = orc.w4_quantize_asym(W, group)
+    assert np.all(z[:, 3] == 0) and np.all(z[:, 4] == 15)
+    assert s[0, 5] == 1.0 and z[0, 5] == 0 and np.all(q[:128, 5] == 0)
+    assert q.max() <= 15 and z.max() <= 15
+    Wd = orc.w4_dequant(orc.w4_pack(q), s, orc.w4_pack_zeros(z), group).astype(np.float32)
+    sc = np.repeat(s.astype(np.float32), group, axis=0)
+    # every value lies inside [lo, hi], so nothing clips: error <= scale / 2
+    # (the fp16 scale may round down by half an ulp, and dequant rounds once)
+    assert np.all(np.abs(W.astype(np.float32) - Wd) <= sc * 0.5 * (1 + 2 ** -10) + np.abs(Wd) * 2 ** -11 + 1e-7)
+    # zero is representable: W = 0 dequantises to exactly 0
+    zero = W == 0
+    assert np.all(Wd[zero] == 0)
+
+
+def test_quantize_asym_round_half_even(orc):
+    # lo = -7, hi = 8: scale (8 + 7) / 15 = 1, z = 7; ties round to even
+    W = np.array([-7.0, 8.0, 0.5, 1.5, 2.5, -0.5, -1.5, 0.0], np.float16)[:, None]
+    q, z, s = orc.w4_quantize_asym(W, 8)
+    assert float(s[0, 0]) == 1.0 and int(z[0, 0]) == 7
+    assert list(q[:, 0]) == [0, 15, 7, 9, 9, 7, 5, 7]
