@@ -151,6 +151,10 @@ def cpu_oracle_routines(budget_s=2.0):
     r = rate(lambda: orc.w4_dequant(packed, sc, qz, 128))
     out["w4_dequant"] = {"weights/s": round(4096 * 11008 * r, 0), "cores": 1,
                          "sample": "4096 x 11008 int4 weights -> FP16"}
+    Wq = Wdq[:1024]
+    r = rate(lambda: orc.w4_quantize_sym(Wq, 128))
+    out["w4_quantize_sym"] = {"weights/s": round(1024 * 11008 * r, 0), "cores": 1,
+                              "sample": "1024 x 11008 fp16 weights -> int4 + scales (quantize.cpp:8-37)"}
     return out
 
 
@@ -315,6 +319,34 @@ def bench_routines(torch, R, timer, dev, quick):
                               "frac_fp16_mfma": round(flops / ms / 1e9 / FP16_PEAK, 4), "max_rel_err_vs_fp64": err,
                               "kernel": w4a16_kind_name(M, Nw, Kw)}
         del Wd, Ar
+    # the int4 quantizer (the producer of the w4a16 weights): W 4096 x 11008
+    # fp16 -> packed int4, scales (and zero-points), over 4 copies of W (360 MB,
+    # more than the Infinity Cache) replayed from a hipGraph
+    Wq = torch.empty(Kw, Nw, dtype=torch.float16, device=dev)
+    R.fill_uniform(Wq, -1.0, 1.0, 1234, inputs.S_W4_Q)
+    copies = 4
+    srcs = [Wq] + [Wq.clone() for _ in range(copies - 1)]
+    for asym in (False, True):
+        outs = [R.w4_quantize(w_, G, asym) for w_ in srcs]
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(torch.cuda.current_stream())
+        K8 = Kw // 8
+        with torch.cuda.stream(side):
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph, stream=side):
+                for w_, (p_, s_, z_) in zip(srcs, outs):
+                    R.call("lthip_w4_quantize", w_.data_ptr(), p_.data_ptr(), s_.data_ptr(), R._ptr(z_), Kw, Nw, G,
+                           1 if asym else 0, R._stream(dev))
+            graph.replay()
+            ms = timer.per_launch_ms(graph.replay, max(4, it // 4)) / copies
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        byts = 2 * Kw * Nw + 4 * K8 * Nw + 2 * (Kw // G) * Nw + (4 * (Kw // G) * (Nw // 8) if asym else 0)
+        out["w4_quantize_" + ("asym" if asym else "sym")] = {
+            "us": round(ms * 1e3, 2), "GB/s": round(byts / ms / 1e6, 1), "frac_hbm": round(byts / ms / 1e6 / HBM_PEAK_GBS, 4),
+            "weights/s": round(Kw * Nw / ms * 1e3, 1), "shape": f"{Kw}x{Nw} fp16, group {G}"}
+        del outs, graph
+    del srcs, Wq
     return out
 
 
