@@ -415,8 +415,8 @@ def main():
                     help="also run 5 plain 256 MiB copies (known traffic, to calibrate HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md on sharding); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (each routine would shard its "
+              "columns or rows across GPUs independently); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
