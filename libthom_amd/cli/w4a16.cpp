@@ -70,8 +70,10 @@ static float w4a16_tiled(const Problem& p) {
 }
 static float w4a16_pipelined(const Problem& p) {
     if (p.N % 32 != 0) return run_cfg(p, nullptr);   // the pipelined tiles need N % 32 == 0
-    const int bm = p.M > 128 ? (p.M > 512 ? 256 : 128) : 128, bn = p.M > 128 ? 256 : 128;
-    lthip_w4a16_config c{3, bn, bm, 1};
+    // the library's choice: 256 x 256 once those tiles alone fill the CUs, else 128 x 128
+    const size_t tiles256 = ((p.M + 255) / 256) * ((p.N + 255) / 256);
+    const int t = p.M > 512 && tiles256 >= 256 ? 256 : 128;
+    lthip_w4a16_config c{3, t, t, 1};
     return run_cfg(p, &c);
 }
 static float w4a16_tuned(const Problem& p) {

@@ -53,6 +53,14 @@ __device__ __forceinline__ uint32_t h2_as_u32(__half2 h) {
     return v;
 }
 
+// v_and_or_b32 with both constants in registers: one VOP3 instruction (with
+// literal constants the compiler splits (x & m) | c into v_and + v_or)
+__device__ __forceinline__ uint32_t and_or(uint32_t x, uint32_t m, uint32_t c) {
+    uint32_t r;
+    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(m), "v"(c));
+    return r;
+}
+
 // Eight int4 weights (k = i at bits 4i of w) -> fp16((q - z) * s), emitted in
 // k order [0,4,1,5,2,6,3,7].  zz holds fp16(1024 + z) in both halves, ss the
 // scale in both halves.  0x6400|q is fp16(1024 + q); the subtraction is exact
@@ -61,10 +69,11 @@ __device__ __forceinline__ uint32_t h2_as_u32(__half2 h) {
 __device__ __forceinline__ uint4 dequant8(uint32_t w, uint32_t zz, uint32_t ss) {
     const __half2 z2 = u32_as_h2(zz), s2 = u32_as_h2(ss);
     uint4 r;
-    r.x = h2_as_u32(__hmul2(__hsub2(u32_as_h2((w & 0x000F000Fu) | 0x64006400u), z2), s2));
-    r.y = h2_as_u32(__hmul2(__hsub2(u32_as_h2(((w >> 4) & 0x000F000Fu) | 0x64006400u), z2), s2));
-    r.z = h2_as_u32(__hmul2(__hsub2(u32_as_h2(((w >> 8) & 0x000F000Fu) | 0x64006400u), z2), s2));
-    r.w = h2_as_u32(__hmul2(__hsub2(u32_as_h2(((w >> 12) & 0x000F000Fu) | 0x64006400u), z2), s2));
+    const uint32_t m = 0x000F000Fu, c = 0x64006400u;
+    r.x = h2_as_u32(__hmul2(__hsub2(u32_as_h2(and_or(w, m, c)), z2), s2));
+    r.y = h2_as_u32(__hmul2(__hsub2(u32_as_h2(and_or(w >> 4, m, c)), z2), s2));
+    r.z = h2_as_u32(__hmul2(__hsub2(u32_as_h2(and_or(w >> 8, m, c)), z2), s2));
+    r.w = h2_as_u32(__hmul2(__hsub2(u32_as_h2(and_or(w >> 12, m, c)), z2), s2));
     return r;
 }
 

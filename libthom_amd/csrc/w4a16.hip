@@ -265,13 +265,6 @@ __device__ __forceinline__ h2v_t as_h2(uint32_t v) {
     __builtin_memcpy(&h, &v, 4);
     return h;
 }
-// v_and_or_b32 with both constants in registers (one VOP3 instruction; the
-// compiler otherwise splits it into v_and + v_or with literal operands)
-__device__ __forceinline__ uint32_t and_or(uint32_t x, uint32_t m, uint32_t c) {
-    uint32_t r;
-    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(m), "v"(c));
-    return r;
-}
 
 constexpr int kDecodeWaves = 8;
 constexpr int kDecodeThreads = 64 * kDecodeWaves;
@@ -791,11 +784,15 @@ void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_
             c.tile_m = 0;
             c.split_k = 1;
         } else if (N % 32 == 0) {
-            // pipelined tiles: the largest tile once M fills it; the launch
-            // shares the last partial round of tiles out by splitting K
+            // pipelined tiles: 128 x 128, or 256 x 256 once those alone fill
+            // every CU (measured: the 128 x 256 tiles were slower than either
+            // at M = 256-512); the launch shares the last partial round of
+            // tiles out by splitting K
             c.kind = 3;
-            c.tile_m = M > 128 ? (M > 512 ? 256 : 128) : 128;
-            c.tile_n = M > 128 ? 256 : 128;
+            const long tiles256 = (long)((M + 255) / 256) * (long)((N + 255) / 256);
+            const bool big = M > 512 && tiles256 >= cu_count();
+            c.tile_m = big ? 256 : 128;
+            c.tile_n = big ? 256 : 128;
             c.split_k = 1;
         } else {
             c.kind = 2;
