@@ -34,11 +34,13 @@ __device__ __forceinline__ int soff(int row, int kc) {
 
 // halves [0..7] -> [0,4,1,5,2,6,3,7]
 __device__ __forceinline__ uint4 perm_k8(uint4 a) {
+    // one v_perm_b32 per word: byte i of the result is byte sel[i] of
+    // {first operand (bytes 4-7), second operand (bytes 0-3)}
     uint4 r;
-    r.x = (a.x & 0xFFFFu) | (a.z << 16);
-    r.y = (a.x >> 16) | (a.z & 0xFFFF0000u);
-    r.z = (a.y & 0xFFFFu) | (a.w << 16);
-    r.w = (a.y >> 16) | (a.w & 0xFFFF0000u);
+    r.x = __builtin_amdgcn_perm(a.z, a.x, 0x05040100u);   // lo(a.x) | lo(a.z) << 16
+    r.y = __builtin_amdgcn_perm(a.z, a.x, 0x07060302u);   // hi(a.x) | hi(a.z) << 16
+    r.z = __builtin_amdgcn_perm(a.w, a.y, 0x05040100u);
+    r.w = __builtin_amdgcn_perm(a.w, a.y, 0x07060302u);
     return r;
 }
 
