@@ -382,7 +382,8 @@ def add_measured_fractions(routines, peaks):
 
 
 def vendor_reference(torch, timer, dev):
-    """torch.matmul (hipBLASLt / rocBLAS) on the hgemm and dgemm shapes."""
+    """torch.matmul (hipBLASLt / rocBLAS) on the hgemm and dgemm shapes, and the
+    FP16 GEMM on dequantised weights that the reference's w4a16 compares with."""
     out = {"note": "vendor library through torch.matmul, for context; not this library"}
     for n in (4096, 8192):
         a = torch.randn(n, n, device=dev, dtype=torch.float16)
@@ -398,8 +399,45 @@ def vendor_reference(torch, timer, dev):
     ms = timer.per_launch_ms(lambda: torch.matmul(a, b), 5)
     out[f"fp64_matmul_{n}"] = {"ms": round(ms, 4), "TFLOP/s": round(2 * n ** 3 / ms / 1e9, 2)}
     del a, b
+    # the reference's w4a16 baseline (w4a16.cpp:306): an FP16 GEMM on the
+    # dequantised weights, K = 4096, N = 11008; M <= 256 timed like the w4a16
+    # rows (back-to-back launches over 4 copies of the 90 MB fp16 weights,
+    # replayed from a hipGraph)
+    try:
+        vendor_w4a16_baseline(torch, timer, dev, out)
+    except Exception as e:   # a context row, never worth losing the run for
+        out["fp16_matmul_w4a16"] = {"error": f"{type(e).__name__}: {e}"[:200]}
     torch.cuda.empty_cache()
     return out
+
+
+def vendor_w4a16_baseline(torch, timer, dev, out):
+    K, N = 4096, 11008
+    w = [torch.randn(K, N, device=dev, dtype=torch.float16) for _ in range(4)]
+    for M in (1, 16, 128, 2048):
+        a = torch.randn(M, K, device=dev, dtype=torch.float16)
+        c = torch.empty(M, N, device=dev, dtype=torch.float16)
+        for w_ in w:
+            torch.matmul(a, w_, out=c)
+        if M <= 256:
+            side = torch.cuda.Stream(device=dev)
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph, stream=side):
+                    for w_ in w:
+                        torch.matmul(a, w_, out=c)
+                graph.replay()
+                ms = timer.per_launch_ms(graph.replay, 10) / len(w)
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            del graph
+        else:
+            ms = timer.per_launch_ms(lambda: torch.matmul(a, w[0], out=c), 10)
+        out[f"fp16_matmul_w4a16_M{M}"] = {"us": round(ms * 1e3, 2), "TFLOP/s": round(2 * M * N * K / ms / 1e9, 2),
+                                          "GB/s": round((2 * K * N + 2 * M * K + 2 * M * N) / ms / 1e6, 1)}
+        del a, c
+    del w
 
 
 def main():
