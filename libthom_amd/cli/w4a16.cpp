@@ -70,9 +70,9 @@ static float w4a16_tiled(const Problem& p) {
 }
 static float w4a16_pipelined(const Problem& p) {
     if (p.N % 32 != 0) return run_cfg(p, nullptr);   // the pipelined tiles need N % 32 == 0
-    // the library's choice: 256 x 256 once those tiles alone fill the CUs, else 128 x 128
+    // the library's choice: 256 x 256 once those tiles cover half the CUs, else 128 x 128
     const size_t tiles256 = ((p.M + 255) / 256) * ((p.N + 255) / 256);
-    const int t = p.M > 512 && tiles256 >= 256 ? 256 : 128;
+    const int t = p.M > 512 && 2 * tiles256 >= 256 ? 256 : 128;
     lthip_w4a16_config c{3, t, t, 1};
     return run_cfg(p, &c);
 }

@@ -132,11 +132,10 @@ struct W4Src {
 
 template <class C>
 struct W4Frags {
-    v8h a[C::TM], b[C::TN];
+    v8h a[C::TM];
+    uint32_t w[C::TN];   // packed int4 words, dequantised where they are used (dq)
 
-    // zz[j] = fp16(1024 + z) in both halves, s2[j] = scale in both halves
-    __device__ __forceinline__ void load(const char* sa, const char* sb, int kk, int wm, int wn, const uint32_t* zz,
-                                         const uint32_t* s2) {
+    __device__ __forceinline__ void load(const char* sa, const char* sb, int kk, int wm, int wn) {
         const int lane = threadIdx.x & 63;
         const int frow = lane & 31, fkc = lane >> 5;
 #pragma unroll
@@ -151,13 +150,20 @@ struct W4Frags {
 #pragma unroll
         for (int j = 0; j < C::TN; j++) {
             const int c = wn + 32 * j + frow;
-            const uint32_t wv =
-                *reinterpret_cast<const uint32_t*>(sb + r * (C::BN * 4) + (((c >> 2) ^ swzW2(r)) << 4) + (c & 3) * 4);
-            const uint4 d = dequant8(wv, zz[j], s2[j]);
-            __builtin_memcpy(&b[j], &d, 16);
+            w[j] = *reinterpret_cast<const uint32_t*>(sb + r * (C::BN * 4) + (((c >> 2) ^ swzW2(r)) << 4) + (c & 3) * 4);
         }
     }
 };
+
+// B fragment j of f: fp16((q - z) * s) in the perm_k8 k order; zz = fp16(1024
+// + z) and s2 = the scale, both in both halves
+template <class C>
+__device__ __forceinline__ v8h dq(const W4Frags<C>& f, int j, const uint32_t* zz, const uint32_t* s2) {
+    const uint4 d = dequant8(f.w[j], zz[j], s2[j]);
+    v8h b;
+    __builtin_memcpy(&b, &d, 16);
+    return b;
+}
 
 // scales and zero-points of this lane's columns for the slab in stage `ss`
 template <class C>
@@ -209,7 +215,7 @@ __device__ __forceinline__ void w4_tile2(char* smem, const __half* __restrict__ 
         uint32_t zz[TN], s2[TN];
         W4Frags<C> fr[2];
         w4_read_sz<C>(smem + C::A_BYTES + C::B_BYTES, cg, sym, zz, s2);
-        fr[0].load(smem, smem + C::A_BYTES, 0, wm, wn, zz, s2);
+        fr[0].load(smem, smem + C::A_BYTES, 0, wm, wn);
         int stage = 0, fill_stage = STAGES - 1;
         for (int s = 0; s < nslabs; s++) {
             const char* sa = smem + stage * C::STAGE_BYTES;
@@ -218,26 +224,33 @@ __device__ __forceinline__ void w4_tile2(char* smem, const __half* __restrict__ 
             __builtin_amdgcn_sched_barrier(0);
             static_for<0, KSTEPS - 1>([&](auto kkc) {
                 constexpr int kk = decltype(kkc)::value;
-                fr[(kk + 1) & 1].load(sa, sb, kk + 1, wm, wn, zz, s2);
+                fr[(kk + 1) & 1].load(sa, sb, kk + 1, wm, wn);
                 src.template issue<kk * C::LOADS / KSTEPS, (kk + 1) * C::LOADS / KSTEPS>(smem, fill_stage, fill);
                 const W4Frags<C>& f = fr[kk & 1];
 #pragma unroll
-                for (int i = 0; i < TM; i++)
+                for (int j = 0; j < TN; j++) {
+                    const v8h bj = dq(f, j, zz, s2);
 #pragma unroll
-                    for (int j = 0; j < TN; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+                    for (int i = 0; i < TM; i++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], bj, acc[i][j], 0, 0, 0);
+                }
                 interleave<MFMAS, READS, (kk + 1) * C::LOADS / KSTEPS - kk * C::LOADS / KSTEPS>();
                 __builtin_amdgcn_sched_barrier(0);
             });
             {
                 constexpr int KL = KSTEPS - 1;
                 const W4Frags<C>& f = fr[KL & 1];
+                // this slab's scales and zero-points are replaced below, so the
+                // last k-step's B fragments are dequantised first
+                v8h bl[TN];
+#pragma unroll
+                for (int j = 0; j < TN; j++) bl[j] = dq(f, j, zz, s2);
                 src.template issue<KL * C::LOADS / KSTEPS, C::LOADS>(smem, fill_stage, fill);
 #pragma unroll
                 for (int i = 0; i < TM / 2; i++)
 #pragma unroll
                     for (int j = 0; j < TN; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], bl[j], acc[i][j], 0, 0, 0);
                 interleave<MFMAS / 2, 0, C::LOADS - KL * C::LOADS / KSTEPS>();
                 __builtin_amdgcn_sched_barrier(0);
                 ring_barrier<C::LOADS * (STAGES - 2)>();
@@ -245,12 +258,12 @@ __device__ __forceinline__ void w4_tile2(char* smem, const __half* __restrict__ 
                 fill_stage = fill_stage + 1 == STAGES ? 0 : fill_stage + 1;
                 const char* na = smem + stage * C::STAGE_BYTES;
                 w4_read_sz<C>(na + C::A_BYTES + C::B_BYTES, cg, sym, zz, s2);
-                fr[0].load(na, na + C::A_BYTES, 0, wm, wn, zz, s2);   // KSTEPS is even
+                fr[0].load(na, na + C::A_BYTES, 0, wm, wn);   // KSTEPS is even
 #pragma unroll
                 for (int i = TM / 2; i < TM; i++)
 #pragma unroll
                     for (int j = 0; j < TN; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], bl[j], acc[i][j], 0, 0, 0);
                 interleave<MFMAS / 2, READS, 0>();
                 __builtin_amdgcn_sched_barrier(0);
             }
