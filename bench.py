@@ -169,6 +169,27 @@ def bench_routines(torch, R, timer, dev, quick):
         R.copy_f32(flush_buf, flush_src)
 
     it = 5 if quick else 20
+    # sgemv on the configured 16384 x 8192 matrix stored column-major (the
+    # reference's col-major layout, sgemv.cpp:180-181): column slices + a
+    # slice-order sum; back-to-back launches over the 537 MB stream
+    m, n = 16384, 8192
+    Ac = torch.empty(n, m, dtype=torch.float32, device=dev)
+    R.fill_uniform(Ac, -1.0, 1.0, 1234, inputs.S_SGEMV_A)
+    xc = torch.empty(n, dtype=torch.float32, device=dev)
+    R.fill_uniform(xc, -1.0, 1.0, 1234, inputs.S_SGEMV_X)
+    yc = torch.zeros(m, dtype=torch.float32, device=dev)
+    one = torch.ones(1, dtype=torch.float32, device=dev)
+    zero = torch.zeros(1, dtype=torch.float32, device=dev)
+    col = lambda: R.sgemv(Ac, xc, yc, row_major=False, alpha_dev=one, beta_dev=zero)
+    col()
+    ms = timer.per_launch_ms(col, it)
+    gbs = 4 * (m * n + n + m) / ms / 1e6
+    Ad, xd = Ac.double(), xc.double()
+    err = ((yc.double() - Ad.t() @ xd).abs() / (Ad.abs().t() @ xd.abs())).max().item()
+    out["sgemv_col_16384x8192"] = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+                                   "max_rel_err_vs_fp64": err}
+    del Ac, Ad, xd
+    torch.cuda.empty_cache()
     # reduction
     for n in (1_000_000, 500_000_000):
         x = torch.empty(n, dtype=torch.float32, device=dev)

@@ -2,13 +2,18 @@
 //
 // Row-major (the reference's sgemv_vecto layout, sgemv_vecto.cu:10-60): one
 // wave per row, x staged once per block into LDS (which also absorbs incx),
-// eight float4 loads of A in flight per lane, DPP wave sum.  Blocks stride over
-// rows so each block stages x once for several rows.
+// eight float4 loads of A in flight per lane (a ragged end of a row as one
+// predicated batch), DPP wave sum.  Blocks stride over rows so each block
+// stages x once for several rows; on large matrices the grid gives every wave
+// the same number of rows with about 1536 waves in all (see row_blocks).
 //
 // Column-major (sgemv_col3_vecto.cu:7-66 computes the same thing): each lane
-// owns four consecutive rows and walks a slice of the columns; column slices
-// write FP32 partials to the workspace and a second kernel adds them in slice
-// order (deterministic, no atomics) and applies alpha/beta.
+// owns four consecutive rows and walks a slice of the columns with eight
+// column loads in flight; column slices write FP32 partials to the workspace
+// and a second kernel adds them in slice order (deterministic, no atomics) and
+// applies alpha/beta.  The slice count aims at about 192 blocks: enough
+// streams to keep HBM busy, few enough that each block walks long runs of
+// columns (more, shorter streams measured slower at every large shape).
 #include "lt_common.hpp"
 
 namespace lt {
@@ -19,7 +24,10 @@ constexpr int kRowBlock = kRowWaves * kWave;
 constexpr int kRowUnroll = 8;
 constexpr int kMaxLdsX = 16384;   // floats of x staged in LDS (64 KB)
 
-template <bool VEC, bool XLDS>
+// TAIL: n/4 is not a multiple of the eight-chunk step, so each row ends with
+// a predicated batch of up to seven chunks (without it, rows whose length is a
+// multiple of the step skip that code: measured faster for short rows)
+template <bool VEC, bool XLDS, bool TAIL>
 __global__ __launch_bounds__(kRowBlock) void sgemv_row_kernel(int m, int n, const float* __restrict__ alpha_p,
                                                               const float* __restrict__ A, long lda,
                                                               const float* __restrict__ x, long incx,
@@ -56,13 +64,22 @@ __global__ __launch_bounds__(kRowBlock) void sgemv_row_kernel(int m, int n, cons
                     acc[u] = fmaf(av[u].w, xv.w, acc[u]);
                 }
             }
-            for (; c < n4; c += kWave) {
-                const float4 av = a4[c];
-                const float4 xv = XLDS ? xs4[c] : xg4[c];
-                acc[0] = fmaf(av.x, xv.x, acc[0]);
-                acc[0] = fmaf(av.y, xv.y, acc[0]);
-                acc[0] = fmaf(av.z, xv.z, acc[0]);
-                acc[0] = fmaf(av.w, xv.w, acc[0]);
+            // fewer than kRowUnroll chunks left: issue them all, then use them
+            if (TAIL) {
+                float4 av[kRowUnroll];
+#pragma unroll
+                for (int u = 0; u < kRowUnroll; u++)
+                    av[u] = c + u * kWave < n4 ? ld_nt(&a4[c + u * kWave]) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int u = 0; u < kRowUnroll; u++) {
+                    if (c + u * kWave < n4) {
+                        const float4 xv = XLDS ? xs4[c + u * kWave] : xg4[c + u * kWave];
+                        acc[u] = fmaf(av[u].x, xv.x, acc[u]);
+                        acc[u] = fmaf(av[u].y, xv.y, acc[u]);
+                        acc[u] = fmaf(av[u].z, xv.z, acc[u]);
+                        acc[u] = fmaf(av[u].w, xv.w, acc[u]);
+                    }
+                }
             }
             const int t = 4 * n4 + lane;
             if (t < n) acc[1] = fmaf(arow[t], XLDS ? xs[t] : x[t], acc[1]);
@@ -79,20 +96,51 @@ __global__ __launch_bounds__(kRowBlock) void sgemv_row_kernel(int m, int n, cons
     }
 }
 
+// Blocks for the row kernel.  Matrices of 128 MB and more, with x in LDS:
+// about 1536 waves
+// (12 MB of loads in flight at eight float4 per lane), every wave given the
+// same number of rows, since the waves with one row more than the others set
+// the time; this measured faster than a grid that fills every CU (fewer,
+// longer streams).  Rows shorter than eight chunks of 64 float4 keep fewer
+// loads in flight per wave and get proportionally more waves.  Smaller
+// matrices are latency-bound, and rows too long for LDS reread x from L2
+// beside A: one row per wave, up to four blocks per CU.
+long row_blocks(int m, int n, bool xlds) {
+    if (!xlds || (double)m * (double)n * 4.0 < 128.0 * 1024 * 1024) {
+        const long blocks = (m + kRowWaves - 1) / kRowWaves;
+        const long cap = (long)cu_count() * 4;
+        return blocks < cap ? blocks : cap;
+    }
+    int chunks = (n / 4 + kWave - 1) / kWave;   // float4 loads per lane and row
+    if (chunks > kRowUnroll) chunks = kRowUnroll;
+    if (chunks < 1) chunks = 1;
+    const long waves = 1536L * kRowUnroll / chunks;
+    long rows = (long)((double)m / (double)waves + 0.5);
+    if (rows < 1) rows = 1;
+    return (m + kRowWaves * rows - 1) / (kRowWaves * rows);
+}
+
 constexpr int kColBlock = 256;
 constexpr int kColRows = kColBlock * 4;   // rows per block (4 per lane)
-constexpr int kColMaxSplit = 64;
+constexpr int kColUnroll = 8;             // column loads in flight per lane
+constexpr int kColTargetBlocks = 192;
+constexpr int kColMinCols = 64;           // columns per slice, at least (when n allows)
+constexpr int kColMaxSplit = 256;
 
 int col_splits(int m, int n) {
-    (void)m;
-    int s = n / 64;
+    const int rowblocks = (m + kColRows - 1) / kColRows;
+    int s = (kColTargetBlocks + rowblocks - 1) / rowblocks;
+    const int cap = n / kColMinCols;
+    if (s > cap) s = cap;
     if (s > kColMaxSplit) s = kColMaxSplit;
     if (s < 1) s = 1;
     return s;
 }
 
-// partial[split][row] = sum over the split's columns of A[row, j] x[j]
-template <bool VEC>
+// partial[split][row] = sum over the split's columns of A[row, j] x[j], in
+// column order.  XU: incx == 1, so the eight x values of a step come from one
+// contiguous scalar load
+template <bool VEC, bool XU>
 __global__ __launch_bounds__(kColBlock) void sgemv_col_partial_kernel(int m, int n, const float* __restrict__ A,
                                                                       long lda, const float* __restrict__ x, long incx,
                                                                       int splits, float* __restrict__ partial) {
@@ -100,40 +148,45 @@ __global__ __launch_bounds__(kColBlock) void sgemv_col_partial_kernel(int m, int
     const int j0 = (int)((long)n * split / splits), j1 = (int)((long)n * (split + 1) / splits);
     const long i0 = (long)blockIdx.x * kColRows + 4L * threadIdx.x;
     if (i0 >= m) return;
-    float4 acc0 = make_float4(0.f, 0.f, 0.f, 0.f), acc1 = acc0;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     const bool full = VEC && (i0 + 3 < m);
     int j = j0;
     if (full) {
-        for (; j + 1 < j1; j += 2) {
-            const float xa = x[(long)j * incx], xb = x[(long)(j + 1) * incx];
-            const float4 a = ld_nt(reinterpret_cast<const float4*>(A + i0 + (long)j * lda));
-            const float4 b = ld_nt(reinterpret_cast<const float4*>(A + i0 + (long)(j + 1) * lda));
-            acc0.x = fmaf(a.x, xa, acc0.x); acc0.y = fmaf(a.y, xa, acc0.y);
-            acc0.z = fmaf(a.z, xa, acc0.z); acc0.w = fmaf(a.w, xa, acc0.w);
-            acc1.x = fmaf(b.x, xb, acc1.x); acc1.y = fmaf(b.y, xb, acc1.y);
-            acc1.z = fmaf(b.z, xb, acc1.z); acc1.w = fmaf(b.w, xb, acc1.w);
+        for (; j + kColUnroll <= j1; j += kColUnroll) {
+            float4 a[kColUnroll];
+            float xs[kColUnroll];
+#pragma unroll
+            for (int u = 0; u < kColUnroll; u++) {
+                xs[u] = XU ? x[j + u] : x[(long)(j + u) * incx];
+                a[u] = ld_nt(reinterpret_cast<const float4*>(A + i0 + (long)(j + u) * lda));
+            }
+#pragma unroll
+            for (int u = 0; u < kColUnroll; u++) {
+                acc.x = fmaf(a[u].x, xs[u], acc.x); acc.y = fmaf(a[u].y, xs[u], acc.y);
+                acc.z = fmaf(a[u].z, xs[u], acc.z); acc.w = fmaf(a[u].w, xs[u], acc.w);
+            }
         }
         for (; j < j1; j++) {
             const float xa = x[(long)j * incx];
             const float4 a = *reinterpret_cast<const float4*>(A + i0 + (long)j * lda);
-            acc0.x = fmaf(a.x, xa, acc0.x); acc0.y = fmaf(a.y, xa, acc0.y);
-            acc0.z = fmaf(a.z, xa, acc0.z); acc0.w = fmaf(a.w, xa, acc0.w);
+            acc.x = fmaf(a.x, xa, acc.x); acc.y = fmaf(a.y, xa, acc.y);
+            acc.z = fmaf(a.z, xa, acc.z); acc.w = fmaf(a.w, xa, acc.w);
         }
     } else {
         for (; j < j1; j++) {
             const float xa = x[(long)j * incx];
             const float* col = A + (long)j * lda;
-            acc0.x = fmaf(col[i0], xa, acc0.x);
-            if (i0 + 1 < m) acc0.y = fmaf(col[i0 + 1], xa, acc0.y);
-            if (i0 + 2 < m) acc0.z = fmaf(col[i0 + 2], xa, acc0.z);
-            if (i0 + 3 < m) acc0.w = fmaf(col[i0 + 3], xa, acc0.w);
+            acc.x = fmaf(col[i0], xa, acc.x);
+            if (i0 + 1 < m) acc.y = fmaf(col[i0 + 1], xa, acc.y);
+            if (i0 + 2 < m) acc.z = fmaf(col[i0 + 2], xa, acc.z);
+            if (i0 + 3 < m) acc.w = fmaf(col[i0 + 3], xa, acc.w);
         }
     }
     float* out = partial + (long)split * m + i0;
-    out[0] = acc0.x + acc1.x;
-    if (i0 + 1 < m) out[1] = acc0.y + acc1.y;
-    if (i0 + 2 < m) out[2] = acc0.z + acc1.z;
-    if (i0 + 3 < m) out[3] = acc0.w + acc1.w;
+    out[0] = acc.x;
+    if (i0 + 1 < m) out[1] = acc.y;
+    if (i0 + 2 < m) out[2] = acc.z;
+    if (i0 + 3 < m) out[3] = acc.w;
 }
 
 __global__ __launch_bounds__(256) void sgemv_col_finish_kernel(int m, int splits, const float* __restrict__ partial,
@@ -156,7 +209,7 @@ __global__ __launch_bounds__(256) void sgemv_col_finish_kernel(int m, int splits
 extern "C" {
 
 size_t lthip_sgemv_f32_workspace_size(int row_major, int m, int n) {
-    if (row_major || m <= 0 || n <= 0) return 0;
+    if (row_major || m <= 0 || n < 0) return 0;
     return sizeof(float) * (size_t)lt::col_splits(m, n) * (size_t)m;
 }
 
@@ -173,25 +226,25 @@ int lthip_sgemv_f32(int row_major, int m, int n, const float* alpha, const float
         const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (lda % 4 == 0);
         const bool xlds = n <= kMaxLdsX;
         const bool xvec_ok = xlds || (incx == 1 && reinterpret_cast<uintptr_t>(x) % 16 == 0);
-        long blocks = (m + kRowWaves - 1) / kRowWaves;
-        const long cap = (long)cu_count() * 4;
-        if (blocks > cap) blocks = cap;
+        const long blocks = row_blocks(m, n, xlds);
         const size_t lds = xlds ? sizeof(float) * (size_t)((n + 3) & ~3) : 0;
+        const bool tail = (n / 4) % (kRowUnroll * kWave) != 0;
+#define LT_ROW(V, X, T)                                                                                              \
+    hipLaunchKernelGGL((sgemv_row_kernel<V, X, T>), dim3((unsigned)blocks), dim3(kRowBlock), lds, s, m, n,         \
+                       alpha, A, (long)lda, x, (long)incx, beta, y, (long)incy)
         if (vec && xvec_ok) {
-            if (xlds)
-                hipLaunchKernelGGL((sgemv_row_kernel<true, true>), dim3((unsigned)blocks), dim3(kRowBlock), lds, s, m, n,
-                                   alpha, A, (long)lda, x, (long)incx, beta, y, (long)incy);
-            else
-                hipLaunchKernelGGL((sgemv_row_kernel<true, false>), dim3((unsigned)blocks), dim3(kRowBlock), 0, s, m, n,
-                                   alpha, A, (long)lda, x, (long)incx, beta, y, (long)incy);
+            if (xlds) {
+                if (tail) LT_ROW(true, true, true);
+                else LT_ROW(true, true, false);
+            } else {
+                if (tail) LT_ROW(true, false, true);
+                else LT_ROW(true, false, false);
+            }
         } else {
-            if (xlds)
-                hipLaunchKernelGGL((sgemv_row_kernel<false, true>), dim3((unsigned)blocks), dim3(kRowBlock), lds, s, m,
-                                   n, alpha, A, (long)lda, x, (long)incx, beta, y, (long)incy);
-            else
-                hipLaunchKernelGGL((sgemv_row_kernel<false, false>), dim3((unsigned)blocks), dim3(kRowBlock), 0, s, m,
-                                   n, alpha, A, (long)lda, x, (long)incx, beta, y, (long)incy);
+            if (xlds) LT_ROW(false, true, false);
+            else LT_ROW(false, false, false);
         }
+#undef LT_ROW
         LT_TRY_LAUNCH("sgemv row launch");
         return LTHIP_OK;
     }
@@ -202,12 +255,15 @@ int lthip_sgemv_f32(int row_major, int m, int n, const float* alpha, const float
     float* partial = reinterpret_cast<float*>(ws);
     dim3 grid((unsigned)((m + kColRows - 1) / kColRows), (unsigned)splits);
     const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (lda % 4 == 0);
-    if (vec)
-        hipLaunchKernelGGL((sgemv_col_partial_kernel<true>), grid, dim3(kColBlock), 0, s, m, n, A, (long)lda, x,
-                           (long)incx, splits, partial);
+    if (vec && incx == 1)
+        hipLaunchKernelGGL((sgemv_col_partial_kernel<true, true>), grid, dim3(kColBlock), 0, s, m, n, A, (long)lda,
+                           x, (long)incx, splits, partial);
+    else if (vec)
+        hipLaunchKernelGGL((sgemv_col_partial_kernel<true, false>), grid, dim3(kColBlock), 0, s, m, n, A, (long)lda,
+                           x, (long)incx, splits, partial);
     else
-        hipLaunchKernelGGL((sgemv_col_partial_kernel<false>), grid, dim3(kColBlock), 0, s, m, n, A, (long)lda, x,
-                           (long)incx, splits, partial);
+        hipLaunchKernelGGL((sgemv_col_partial_kernel<false, false>), grid, dim3(kColBlock), 0, s, m, n, A, (long)lda,
+                           x, (long)incx, splits, partial);
     LT_TRY_LAUNCH("sgemv col partial launch");
     hipLaunchKernelGGL(sgemv_col_finish_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, splits, partial,
                        alpha, beta, y, (long)incy);

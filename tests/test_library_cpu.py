@@ -56,7 +56,11 @@ def test_workspace_queries():
     lib = libthom_amd.lib()
     assert lib.lthip_reduce_sum_f32_workspace_size(10 ** 9) >= 256
     assert lib.lthip_sgemv_f32_workspace_size(1, 100, 100) == 0
-    assert lib.lthip_sgemv_f32_workspace_size(0, 16384, 8192) == 64 * 16384 * 4
+    # column-major: 16 row blocks x 12 column slices (about 192 blocks)
+    assert lib.lthip_sgemv_f32_workspace_size(0, 16384, 8192) == 12 * 16384 * 4
+    # few columns cap the slices at n / 64; n = 0 still needs one slice of partials
+    assert lib.lthip_sgemv_f32_workspace_size(0, 1024, 1024) == 16 * 1024 * 4
+    assert lib.lthip_sgemv_f32_workspace_size(0, 5, 0) == 5 * 4
     cfg = _lib.HgemmConfig(128, 128, 64, 4, 2)
     assert lib.lthip_hgemm_workspace_size(512, 512, 4096, ctypes.byref(cfg)) >= 4 * 512 * 512 * 4
     n = lib.lthip_hgemm_config_count()
