@@ -350,8 +350,11 @@ def measured_peaks(device=None, read_bytes: int = 2 << 30) -> dict:
     x = torch.empty(n, dtype=torch.float32, device=device)
     fill_uniform(x, 0.0, 1.0, 1, 99)
     best = 0.0
-    for mult in (2, 4, 8):
-        ms = timed(lambda: call("lthip_probe_read_f32", x.data_ptr(), n, sink.data_ptr(), cus * mult, stream))
+    # grid-stride widths: the stride between a thread's loads (blocks x 4 KiB)
+    # decides how the concurrent requests spread over HBM banks, so the best
+    # of several widths is the device's read rate
+    for blocks in (cus * 3 // 2, cus * 2, cus * 3, cus * 4, cus * 8):
+        ms = timed(lambda: call("lthip_probe_read_f32", x.data_ptr(), n, sink.data_ptr(), blocks, stream))
         best = max(best, n * 4 / ms / 1e6)
     out["hbm_read_gbs"] = round(best, 1)
     y = torch.empty_like(x)
