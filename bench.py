@@ -340,6 +340,11 @@ def bench_routines(torch, R, timer, dev, quick):
                               "frac_fp16_mfma": round(flops / ms / 1e9 / FP16_PEAK, 4), "max_rel_err_vs_fp64": err,
                               "kernel": w4a16_kind_name(M, Nw, Kw)}
         del Wd, Ar
+    # grouped w4a16: many problems, each with its own weights, in one
+    # persistent launch; against the same problems as separate automatic
+    # launches, both replayed from a hipGraph (copies of the problem set rotate
+    # the weights through HBM)
+    out.update(bench_w4a16_grouped(torch, R, inputs, np, timer, dev, it))
     # the int4 quantizer (the producer of the w4a16 weights): W 4096 x 11008
     # fp16 -> packed int4, scales (and zero-points), over 4 copies of W (360 MB,
     # more than the Infinity Cache) replayed from a hipGraph
@@ -368,6 +373,73 @@ def bench_routines(torch, R, timer, dev, quick):
             "weights/s": round(Kw * Nw / ms * 1e3, 1), "shape": f"{Kw}x{Nw} fp16, group {G}"}
         del outs, graph
     del srcs, Wq
+    return out
+
+
+def w4a16_grouped_sets(np):
+    """The grouped w4a16 workloads: (name, [(M, N, K, group)], copies)."""
+    rng = np.random.default_rng(1234)
+    experts = np.maximum(1, np.round(np.exp(rng.uniform(0.0, np.log(512), 32)))).astype(int)
+    return [("w4a16_grouped_configured", [(M, 11008, 4096, 128) for M in (1, 16, 128, 2048)], 3),
+            ("w4a16_grouped_32_experts", [(int(M), 2048, 4096, 128) for M in experts], 4)]
+
+
+def bench_w4a16_grouped(torch, R, inputs, np, timer, dev, it):
+    out = {}
+    for name, dims, copies in w4a16_grouped_sets(np):
+        sets = []
+        for c in range(copies):
+            probs = []
+            for i, (M, N, K, G) in enumerate(dims):
+                A, q, s, z = inputs.w4a16_problem(M, N, K, G, seed=1234 + 97 * c + i)
+                probs.append((torch.from_numpy(A).to(dev), torch.from_numpy(inputs.pack_w4(q).view(np.int32)).to(dev),
+                              torch.from_numpy(s).to(dev),
+                              torch.from_numpy(inputs.pack_zeros(z).view(np.int32)).to(dev),
+                              torch.empty(M, N, dtype=torch.float16, device=dev), G))
+            sets.append(probs)
+        plans = [R.GroupedW4A16(p) for p in sets]
+        separate = [lambda a=a, w=w, s=s, z=z, C=C, G=G: R.w4a16(a, w, s, z, G, C)
+                    for p in sets for (a, w, s, z, C, G) in p]
+
+        def graph_us(fns):
+            side = torch.cuda.Stream(device=dev)
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                for f in fns:
+                    f()
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=side):
+                    for f in fns:
+                        f()
+                g.replay()
+                ms = timer.per_launch_ms(g.replay, max(4, it // 4))
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            return ms * 1e3 / copies
+
+        us_g = graph_us([p.run for p in plans])
+        us_s = graph_us(separate)
+        # the grouped result against an FP64 product of sampled rows of each problem
+        for p in plans:
+            p.run()
+        torch.cuda.synchronize()
+        err = 0.0
+        for (a, w, s, z, C, G) in sets[0]:
+            rows = torch.arange(0, a.shape[0], max(1, a.shape[0] // 16), device=dev)
+            Wd = R.w4_dequant(w, s, z, G).double()
+            Ar = a[rows].double()
+            err = max(err, ((C[rows].double() - Ar @ Wd).abs() / (Ar.abs() @ Wd.abs())).max().item())
+        flops = sum(2 * M * N * K for M, N, K, _ in dims)
+        out[name] = {"us": round(us_g, 1), "TFLOP/s": round(flops / us_g / 1e6, 1),
+                     "frac_fp16_mfma": round(flops / us_g / 1e6 / FP16_PEAK, 4),
+                     "separate_launches_us": round(us_s, 1), "speedup_vs_separate": round(us_s / us_g, 2),
+                     "problems": len(dims), "M": [d[0] for d in dims], "N": dims[0][1], "K": dims[0][2],
+                     "launch": "one persistent kernel in a hipGraph", **plans[0].info(),
+                     "max_rel_err_vs_fp64": err}
+        for p in plans:
+            p.close()
+        del sets, plans
+        torch.cuda.empty_cache()
     return out
 
 

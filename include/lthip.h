@@ -208,6 +208,41 @@ size_t lthip_w4a16_workspace_size(size_t M, size_t N, size_t K, const lthip_w4a1
 int lthip_w4a16_gemm(const uint16_t *A, const uint32_t *W_packed, const uint16_t *scales,
                      const uint32_t *qzeros, uint16_t *C, size_t M, size_t N, size_t K, int group,
                      const lthip_w4a16_config *cfg, void *ws, size_t ws_bytes, void *stream);
+/* ---- grouped / batched w4a16 (no counterpart in the reference, whose
+ *      w4a16 takes one problem per call, w4a16.cpp:247): `count` independent
+ *      problems, each with its own A, weights, scales and zero-points in the
+ *      layout above, in ONE persistent launch.  Each problem takes a pipelined
+ *      tile of 256 columns and 256, 128 or 64 rows: by its M (> 128, > 64,
+ *      else) when cfg is NULL or kind 0, or the one tile of a kind-3 cfg
+ *      (tile_n 256, tile_m 256/128/64, split_k 1) for every problem.  Per
+ *      problem: K > 0, group % 64 == 0, K % group == 0, N % 32 == 0; A, W,
+ *      scales, zero-points 16-byte and C 8-byte aligned; M == 0 or N == 0 is
+ *      skipped.  plan_create uploads the problem table into the workspace
+ *      (blocking, once); run launches it; capture records that launch into a
+ *      hipGraph on a non-null stream and graph_launch replays it.  The launch
+ *      leaves its work queue at zero, so plans and graphs replay as is.
+ *      Problems sharing an output must not be in one plan. */
+typedef struct lthip_w4a16_problem {
+    const uint16_t *A;          /* M x K fp16 */
+    const uint32_t *W_packed;   /* (K/8) x N */
+    const uint16_t *scales;     /* (K/group) x N fp16 */
+    const uint32_t *qzeros;     /* (K/group) x (N/8), or NULL: zero-point 8 */
+    uint16_t *C;                /* M x N fp16 */
+    int M, N, K, group;
+} lthip_w4a16_problem;
+typedef struct lthip_w4a16_grouped_plan lthip_w4a16_grouped_plan;
+
+size_t lthip_w4a16_grouped_workspace_size(const lthip_w4a16_problem *probs, int count,
+                                          const lthip_w4a16_config *cfg);
+int lthip_w4a16_grouped_plan_create(const lthip_w4a16_problem *probs, int count,
+                                    const lthip_w4a16_config *cfg, void *ws, size_t ws_bytes,
+                                    void *stream, lthip_w4a16_grouped_plan **plan);
+int lthip_w4a16_grouped_run(lthip_w4a16_grouped_plan *plan, void *stream);
+int lthip_w4a16_grouped_capture(lthip_w4a16_grouped_plan *plan, void *stream);
+int lthip_w4a16_grouped_graph_launch(lthip_w4a16_grouped_plan *plan, void *stream);
+int lthip_w4a16_grouped_plan_info(const lthip_w4a16_grouped_plan *plan, int *work_items, int *grid);
+int lthip_w4a16_grouped_plan_destroy(lthip_w4a16_grouped_plan *plan);
+
 /* Dequantise the whole weight matrix: W (K x N fp16) -- the bit-exact format
  * check of dequantize_w4 (quantize.cpp:58-65) on the GPU. */
 int lthip_w4_dequant(const uint32_t *W_packed, const uint16_t *scales, const uint32_t *qzeros,

@@ -50,6 +50,11 @@ class W4A16Config(ctypes.Structure):
         return f"W4A16Config(kind={self.kind}, tile_n={self.tile_n}, tile_m={self.tile_m}, split_k={self.split_k})"
 
 
+class W4A16Problem(ctypes.Structure):
+    _fields_ = [("A", c_vp), ("W_packed", c_vp), ("scales", c_vp), ("qzeros", c_vp), ("C", c_vp), ("M", c_int),
+                ("N", c_int), ("K", c_int), ("group", c_int)]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "lthip_status_string": (ctypes.c_char_p, [c_int]),
@@ -91,6 +96,14 @@ _SIGS = {
     "lthip_hgemm_grouped_graph_launch": (c_int, [c_vp, c_vp]),
     "lthip_hgemm_grouped_plan_info": (c_int, [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
     "lthip_hgemm_grouped_plan_destroy": (c_int, [c_vp]),
+    "lthip_w4a16_grouped_workspace_size": (c_size, [c_vp, c_int, ctypes.POINTER(W4A16Config)]),
+    "lthip_w4a16_grouped_plan_create": (c_int, [c_vp, c_int, ctypes.POINTER(W4A16Config), c_vp, c_size, c_vp,
+                                                ctypes.POINTER(c_vp)]),
+    "lthip_w4a16_grouped_run": (c_int, [c_vp, c_vp]),
+    "lthip_w4a16_grouped_capture": (c_int, [c_vp, c_vp]),
+    "lthip_w4a16_grouped_graph_launch": (c_int, [c_vp, c_vp]),
+    "lthip_w4a16_grouped_plan_info": (c_int, [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+    "lthip_w4a16_grouped_plan_destroy": (c_int, [c_vp]),
     "lthip_mlp_forward": (c_int, [c_vp] * 7 + [c_int] * 4 + [c_vp]),
     "lthip_mlp_workspace_size": (c_size, [c_int, c_int, c_int]),
     "lthip_mlp_loss_grad": (c_int, [c_vp] * 7 + [c_int] * 4 + [c_vp] * 5 + [c_vp, c_size, c_vp]),

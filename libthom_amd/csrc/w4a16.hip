@@ -739,7 +739,7 @@ struct Tiled2Entry {
     {bm, bn, Core2W4<bm, bn, st, wm, wn>::SMEM, w4a16_tiled2_kernel<Core2W4<bm, bn, st, wm, wn>>, \
      Core2W4<bm, bn, st, wm, wn>::THREADS},
 const Tiled2Entry kTiled2[] = {LT_W4T2(256, 256, 3, 2, 4) LT_W4T2(256, 128, 3, 2, 2) LT_W4T2(128, 256, 3, 2, 4)
-                                   LT_W4T2(128, 128, 3, 2, 2)};
+                                   LT_W4T2(128, 128, 3, 2, 2) LT_W4T2(64, 256, 3, 1, 8)};
 #undef LT_W4T2
 constexpr int kNumTiled2 = sizeof(kTiled2) / sizeof(kTiled2[0]);
 

@@ -316,6 +316,65 @@ def batched_problem_dims(count: int = 256, seed: int = 1234):
     return [(int(d[3 * i]), int(d[3 * i + 1]), int(d[3 * i + 2])) for i in range(count)]
 
 
+class GroupedW4A16:
+    """A plan for many independent w4a16 problems in one persistent launch.
+
+    problems: list of (A, W_packed, scales, qzeros or None, C, group) tensors
+    in the layout of w4a16() (C written in place); config: None (each problem
+    takes the 256/128/64-row tile by its M) or a kind-3 W4A16Config with
+    tile_n 256 forcing one tile.  run() launches on the current stream;
+    capture(stream) records the launch into a hipGraph on a side stream and
+    replay(stream) relaunches it."""
+
+    def __init__(self, problems, config: Optional[W4A16Config] = None):
+        from ._lib import W4A16Problem
+        _need_cuda(*[t for prob in problems for t in prob[:5]])
+        self._keep = problems
+        n = len(problems)
+        arr = (W4A16Problem * n)()
+        self.device = problems[0][0].device
+        for i, (A, W, S, Z, C, group) in enumerate(problems):
+            M, K = A.shape
+            N = W.shape[1]
+            assert W.shape[0] * 8 == K and C.shape == (M, N) and A.is_contiguous() and C.is_contiguous()
+            arr[i] = W4A16Problem(A.data_ptr(), W.data_ptr(), S.data_ptr(), _ptr(Z), C.data_ptr(), M, N, K, group)
+        self._arr = arr
+        self.config = config
+        cfg = ctypes.byref(config) if config is not None else None
+        need = lib().lthip_w4a16_grouped_workspace_size(ctypes.byref(arr), n, cfg)
+        self.ws = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
+        call("lthip_workspace_clear", self.ws.data_ptr(), self.ws.numel(), _stream(self.device))
+        self._plan = ctypes.c_void_p()
+        call("lthip_w4a16_grouped_plan_create", ctypes.byref(arr), n, cfg, self.ws.data_ptr(), self.ws.numel(),
+             _stream(self.device), ctypes.byref(self._plan))
+
+    def info(self):
+        w, g = ctypes.c_int(), ctypes.c_int()
+        call("lthip_w4a16_grouped_plan_info", self._plan, ctypes.byref(w), ctypes.byref(g))
+        return {"work_items": w.value, "grid": g.value}
+
+    def run(self, stream: Optional[torch.cuda.Stream] = None):
+        s = (stream or torch.cuda.current_stream(self.device)).cuda_stream
+        call("lthip_w4a16_grouped_run", self._plan, s)
+
+    def capture(self, stream: torch.cuda.Stream):
+        call("lthip_w4a16_grouped_capture", self._plan, stream.cuda_stream)
+
+    def replay(self, stream: torch.cuda.Stream):
+        call("lthip_w4a16_grouped_graph_launch", self._plan, stream.cuda_stream)
+
+    def close(self):
+        if self._plan:
+            lib().lthip_w4a16_grouped_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def measured_peaks(device=None, read_bytes: int = 2 << 30) -> dict:
     """What this device delivers: dense MFMA TFLOP/s (fp16, fp8, fp64; best
     of a few launch widths, lthip_probe_mfma) and HBM read and copy GB/s
