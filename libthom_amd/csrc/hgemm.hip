@@ -96,19 +96,31 @@ int find_tile2(int bm, int bn, int bk, int stages) {
 
 // default (no tuner): large tiles for large problems, split-K when the tile
 // grid cannot fill the chip and K is long
+// The automatic tile: the largest DMA-ring tile that still gives about half
+// of the CUs a tile of their own, whole tiles (no split-K).  Measured with
+// settled clocks over square sizes 512-6144: 256 x 256 x 32 from 3072 up,
+// 256 x 128 x 64 at 2560, 128 x 128 x 64 at 2048 and below, each 1.3-3.6x
+// the split-K launches of the larger tiles.  M or N below 512: the 64 x 64
+// register-staged tile, split over K until the CUs are about half full.
 void default_config(long M, long N, long K, lthip_hgemm_config* c) {
-    const int cus = cu_count();
-    c->stages = 2;
-    if (M >= 2048 && N >= 2048) {
-        c->bm = 256; c->bn = 256; c->bk = 32; c->stages = 4;
-    } else if (M >= 512 && N >= 512) {
-        c->bm = 128; c->bn = 128; c->bk = 32; c->stages = 4;
-    } else {
-        c->bm = 64; c->bn = 64; c->bk = 64;
+    const long cus = cu_count();
+    const long need = cus * 9 / 16;
+    auto tiles = [&](long bm, long bn) { return ((M + bm - 1) / bm) * ((N + bn - 1) / bn); };
+    c->split_k = 1;
+    if (M >= 512 && N >= 512) {
+        if (tiles(256, 256) >= need) {
+            c->bm = 256; c->bn = 256; c->bk = 32; c->stages = 4;
+        } else if (tiles(256, 128) >= need) {
+            c->bm = M >= N ? 256 : 128; c->bn = M >= N ? 128 : 256; c->bk = 64; c->stages = 3;
+        } else {
+            c->bm = 128; c->bn = 128; c->bk = 64; c->stages = 3;
+        }
+        return;
     }
-    const long tiles = ((M + c->bm - 1) / c->bm) * ((N + c->bn - 1) / c->bn);
+    c->bm = 64; c->bn = 64; c->bk = 64; c->stages = 2;
+    const long t = tiles(64, 64);
     int split = 1;
-    while (split < 16 && tiles * split * 2 <= cus && K / (c->bk * split * 2) >= 4) split *= 2;
+    while (split < 16 && t * split * 2 <= cus && K / (c->bk * split * 2) >= 4) split *= 2;
     c->split_k = split;
 }
 

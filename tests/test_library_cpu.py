@@ -75,3 +75,19 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(RuntimeError, match="not built"):
         _lib.lib()
+
+
+def test_hgemm_default_config_rule():
+    # without a GPU the library assumes 256 CUs (MI355X): the largest ring
+    # tile that gives at least 144 CUs a tile, no split-K; 64 x 64 with
+    # split-K below 512 rows or columns
+    lib = libthom_amd.lib()
+    want = {(8192, 8192, 8192): (256, 256, 32, 1, 4), (4096, 4096, 4096): (256, 256, 32, 1, 4),
+            (3072, 3072, 3072): (256, 256, 32, 1, 4), (2560, 2560, 2560): (256, 128, 64, 1, 3),
+            (2048, 3072, 2048): (128, 256, 64, 1, 3), (2048, 2048, 2048): (128, 128, 64, 1, 3),
+            (1024, 1024, 1024): (128, 128, 64, 1, 3), (512, 512, 512): (128, 128, 64, 1, 3),
+            (2560, 8192, 1024): (256, 256, 32, 1, 4), (128, 4096, 4096): (64, 64, 64, 2, 2)}
+    for (M, N, K), cfg in want.items():
+        c = _lib.HgemmConfig()
+        assert lib.lthip_hgemm_default_config(M, N, K, ctypes.byref(c)) == 0
+        assert c.key() == cfg, ((M, N, K), c)
