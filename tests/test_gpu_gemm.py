@@ -21,7 +21,8 @@ def t(a, dev):
 
 
 @pytest.mark.parametrize("M,N,K", [(128, 128, 16), (256, 384, 512), (1, 1, 1), (37, 53, 71), (130, 129, 33),
-                                   (200, 17, 1000), (64, 300, 0)])
+                                   (200, 17, 1000), (64, 300, 0),
+                                   (2048, 2304, 64), (2560, 2048, 16)])   # 256 x 128 tiles (>= 144 of them)
 def test_dgemm_small_integers_exact(dev, orc, M, N, K):
     rng = np.random.default_rng(M + N + K)
     A = rng.integers(-2, 3, (M, K)).astype(np.float64)
