@@ -35,6 +35,7 @@ static const std::unordered_map<std::string, RoutineHandler> routinesMap = {
 #endif
 #ifdef LIB_HAS_ROUTINE_W4A16
     {"w4a16", w4a16_handler},
+    {"w4a16_grouped", w4a16_grouped_handler},
 #endif
 };
 

@@ -48,6 +48,7 @@ void dgemm_handler(RunMode runMode, int argc, char* argv[]);
 void hgemm_handler(RunMode runMode, int argc, char* argv[]);
 void hgemm_batched_handler(RunMode runMode, int argc, char* argv[]);
 void w4a16_handler(RunMode runMode, int argc, char* argv[]);
+void w4a16_grouped_handler(RunMode runMode, int argc, char* argv[]);
 
 namespace cli {
 

@@ -358,6 +358,99 @@ static void tune(int argc, char* argv[]) {
 
 }  // namespace w4a16
 
+// ---- grouped w4a16 (no counterpart in the reference): `experts` problems of
+// N x K int4 weights each, M per problem log-uniform in [1, 512] from the
+// seeded generator, all in one persistent launch (or its hipGraph).  Inputs
+// come from the device generator: the check compares every problem bit for
+// bit with the single-problem pipelined tile of the same shape run alone.
+void w4a16_grouped_handler(RunMode runMode, int argc, char* argv[]) {
+    static const std::unordered_map<std::string, bool> impls = {{"persistent", false}, {"graph", true}};
+    if (runMode == RunMode::TUNE || argc < 4 || !impls.count(argv[3])) {
+        fprintf(stderr,
+                "Usage: ./libthom <test|bench> w4a16_grouped <persistent|graph> [experts=32 [N=2048 [K=4096 "
+                "[nb_iter=10]]]]\n");
+        exit(1);
+    }
+    const bool graph = impls.at(argv[3]);
+    const int count = argc > 4 ? atoi(argv[4]) : 32;
+    const int N = argc > 5 ? atoi(argv[5]) : 2048;
+    const int K = argc > 6 ? atoi(argv[6]) : 4096;
+    const int iters = argc > 7 ? atoi(argv[7]) : 10;
+    const int group = GROUP_SIZE;
+    if (count <= 0 || iters <= 0 || N <= 0 || K <= 0 || N % 32 || K % group) {
+        fprintf(stderr, "need positive counts, N %% 32 == 0 and K %% %d == 0\n", group);
+        exit(1);
+    }
+    std::vector<cli::DevBuf<uint16_t>*> h16;
+    std::vector<cli::DevBuf<uint32_t>*> h32;
+    std::vector<lthip_w4a16_problem> probs;
+    double flops = 0.0;
+    for (int i = 0; i < count; i++) {
+        const double u = cli::rng_unit(cli::kSeed, 23, (uint64_t)i);
+        const int M = std::max(1, (int)std::nearbyint(std::exp(std::log(512.0) * u)));
+        auto* A = new cli::DevBuf<uint16_t>((size_t)M * K);
+        auto* S = new cli::DevBuf<uint16_t>((size_t)(K / group) * N);
+        auto* C = new cli::DevBuf<uint16_t>((size_t)M * N);
+        auto* W = new cli::DevBuf<uint32_t>((size_t)(K / 8) * N);
+        auto* Z = new cli::DevBuf<uint32_t>((size_t)(K / group) * (N / 8));
+        h16.insert(h16.end(), {A, S, C});
+        h32.insert(h32.end(), {W, Z});
+        LTHIP_CHECK(lthip_fill_uniform_f16(A->p, A->n, -1.0, 1.0, cli::kSeed, 3000 + 4 * i, cli::stream()));
+        LTHIP_CHECK(lthip_fill_uniform_f16(S->p, S->n, 0.005, 0.02, cli::kSeed, 3001 + 4 * i, cli::stream()));
+        // any 32-bit pattern is 8 valid nibbles: the weights and zero-points
+        // take the bits of uniform FP32 values
+        LTHIP_CHECK(lthip_fill_uniform_f32(reinterpret_cast<float*>(W->p), W->n, 0.0, 1.0, cli::kSeed, 3002 + 4 * i,
+                                           cli::stream()));
+        LTHIP_CHECK(lthip_fill_uniform_f32(reinterpret_cast<float*>(Z->p), Z->n, 0.0, 1.0, cli::kSeed, 3003 + 4 * i,
+                                           cli::stream()));
+        probs.push_back(lthip_w4a16_problem{A->p, W->p, S->p, Z->p, C->p, M, N, K, group});
+        flops += 2.0 * M * N * K;
+    }
+    const size_t need = lthip_w4a16_grouped_workspace_size(probs.data(), count, nullptr);
+    void* ws = cli::workspace("w4a16_grouped", need);
+    lthip_w4a16_grouped_plan* plan = nullptr;
+    LTHIP_CHECK(lthip_w4a16_grouped_plan_create(probs.data(), count, nullptr, ws, need, cli::stream(), &plan));
+    if (graph) LTHIP_CHECK(lthip_w4a16_grouped_capture(plan, cli::stream()));
+    auto once = [&] {
+        if (graph)
+            LTHIP_CHECK(lthip_w4a16_grouped_graph_launch(plan, cli::stream()));
+        else
+            LTHIP_CHECK(lthip_w4a16_grouped_run(plan, cli::stream()));
+    };
+    once();
+    const float ms = cli::time_ms([&] {
+        for (int r = 0; r < iters; r++) once();
+    }) / iters;
+    int work = 0, grid = 0;
+    LTHIP_CHECK(lthip_w4a16_grouped_plan_info(plan, &work, &grid));
+    LTHIP_CHECK(lthip_w4a16_grouped_plan_destroy(plan));
+    if (runMode == RunMode::TEST) {
+        int bad = 0;
+        for (const auto& p : probs) {
+            const lthip_w4a16_config c{3, 256, p.M > 128 ? 256 : (p.M > 64 ? 128 : 64), 1};
+            cli::DevBuf<uint16_t> ref((size_t)p.M * N);
+            // no workspace: whole tiles, the same work as a grouped item
+            LTHIP_CHECK(lthip_w4a16_gemm(p.A, p.W_packed, p.scales, p.qzeros, ref.p, p.M, N, K, group, &c, nullptr,
+                                         0, cli::stream()));
+            std::vector<uint16_t> x(ref.n), y(ref.n);
+            ref.download(x.data());
+            HIP_CHECK(hipMemcpy(y.data(), p.C, sizeof(uint16_t) * y.size(), hipMemcpyDeviceToHost));
+            bad += memcmp(x.data(), y.data(), sizeof(uint16_t) * x.size()) != 0;
+        }
+        if (bad) {
+            printf("ERROR: %d of %d problems differ from the single-problem tile\n", bad, count);
+            cli::g_check_failed = true;
+        }
+    }
+    for (auto* b : h16) delete b;
+    for (auto* b : h32) delete b;
+    const double tf = flops / (ms * 1e9);
+    printf("# %d problems (N %d, K %d, M log-uniform 1-512), %d work items on %d blocks: %.3f ms, %.1f TFLOP/s "
+           "(%.1f%% of FP16 MFMA peak)\n",
+           count, N, K, work, grid, ms, tf, 100.0 * tf / cli::kFp16MfmaTflops);
+    printf("# GFLOP/s: %f\n", tf * 1000.0);
+}
+
 void w4a16_handler(RunMode runMode, int argc, char* argv[]) {
     switch (runMode) {
         case RunMode::TEST:

@@ -72,6 +72,17 @@ def test_cli_test_mode_passes_checks(args):
 
 
 @pytest.mark.gpu
+def test_cli_w4a16_grouped():
+    # every problem bit-identical to the single-problem tile run alone
+    for impl in ("persistent", "graph"):
+        r = run("test", "w4a16_grouped", impl, 12, 544, 1024, 2)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        assert "ERROR" not in r.stdout and "# GFLOP/s" in r.stdout
+    r = run("bench", "w4a16_grouped", "bogus")
+    assert r.returncode != 0
+
+
+@pytest.mark.gpu
 def test_cli_batched_and_tune(tmp_path):
     r = run("test", "hgemm_batched", "persistent", 16, 2)
     assert r.returncode == 0, r.stdout + r.stderr
