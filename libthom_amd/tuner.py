@@ -431,8 +431,10 @@ def batched_problems(dims, device, split_policy: int, cfg, tensors=None):
     return probs, tensors
 
 
-def time_batched(probs, cfg, device, replays: int = 10) -> float:
-    """Median ms per graph replay of the grouped launch."""
+def time_batched(probs, cfg, device, replays: int = 10, warm_ms: float = 100.0) -> float:
+    """Median ms per graph replay of the grouped launch, after at least
+    warm_ms of untimed replays (under a new MFMA load the clocks dip for tens
+    of milliseconds and recover, which would favour the later candidates)."""
     import torch
     from . import routines as R
     plan = R.GroupedHgemm(probs, cfg)
@@ -442,7 +444,16 @@ def time_batched(probs, cfg, device, replays: int = 10) -> float:
         plan.capture(side)
         times = []
         with torch.cuda.stream(side):
-            plan.replay(side)
+            spent = 0.0
+            while spent < warm_ms:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(side)
+                for _ in range(8):
+                    plan.replay(side)
+                e1.record(side)
+                e1.synchronize()
+                spent += max(e0.elapsed_time(e1), 0.01)
             for _ in range(replays):
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
