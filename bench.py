@@ -314,7 +314,13 @@ def bench_routines(torch, R, timer, dev, quick):
     del probs
     torch.cuda.empty_cache()
     # w4a16
+    from libthom_amd import W4A16Config
     Nw, Kw, G = 11008, 4096, 128
+    tuned = {}
+    try:
+        w4_model = tuner.CostModel.load(os.path.join(tuner.TUNING_DIR, "w4a16_mlp.json"), device=str(dev))
+    except Exception:   # no trained model: the tuned rows are left out
+        w4_model = None
     for M in (1, 16, 128, 2048):
         Aw, q, s, z = inputs.w4a16_problem(M, Nw, Kw, G)
         At = torch.from_numpy(Aw).to(dev)
@@ -325,30 +331,53 @@ def bench_routines(torch, R, timer, dev, quick):
         R.w4a16(At, Wt, St, Zt, G, Ct)
         # one call on cold caches, HIP events around it (includes the launch)
         ms_call = timer.per_launch_ms(lambda: R.w4a16(At, Wt, St, Zt, G, Ct), it, flush)
-        if M > 256:
-            # compute-bound: back-to-back launches after the clocks settle
-            ms = timer.steady_ms(lambda: R.w4a16(At, Wt, St, Zt, G, Ct), it)
-        else:
-            # back-to-back launches over 24 copies of the weights (560 MB, more
-            # than the 256 MB Infinity Cache), replayed from a hipGraph so the
-            # host does not pace the ~10 us kernels: the kernel's own rate
-            # with the weights streamed from HBM
-            copies = 24
-            ws = [(Wt, St, Zt)] + [(Wt.clone(), St.clone(), Zt.clone()) for _ in range(copies - 1)]
+        # M > 256, compute-bound: back-to-back launches after the clocks
+        # settle.  M <= 256: back-to-back launches over 24 copies of the
+        # weights (560 MB, more than the 256 MB Infinity Cache), replayed from
+        # a hipGraph so the host does not pace the ~10 us kernels: the kernel's
+        # own rate with the weights streamed from HBM
+        ws = [(Wt, St, Zt)] + ([(Wt.clone(), St.clone(), Zt.clone()) for _ in range(23)] if M <= 256 else [])
+
+        def time_cfg(cfg, warm_ms=100.0):
+            if M > 256:
+                return timer.steady_ms(lambda: R.w4a16(At, Wt, St, Zt, G, Ct, config=cfg), it, warm_ms)
             side = torch.cuda.Stream(device=dev)
             side.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(side):
                 for w_, s_, z_ in ws:
-                    R.w4a16(At, w_, s_, z_, G, Ct)
+                    R.w4a16(At, w_, s_, z_, G, Ct, config=cfg)
                 graph = torch.cuda.CUDAGraph()
                 with torch.cuda.graph(graph, stream=side):
                     for w_, s_, z_ in ws:
-                        R.w4a16(At, w_, s_, z_, G, Ct)
+                        R.w4a16(At, w_, s_, z_, G, Ct, config=cfg)
                 graph.replay()
-                ms = timer.steady_ms(graph.replay, max(4, it // 4)) / copies
+                t = timer.steady_ms(graph.replay, max(4, it // 4), warm_ms) / len(ws)
             torch.cuda.current_stream().wait_stream(side)
             torch.cuda.synchronize()
-            del ws, graph
+            del graph
+            return t
+
+        ms = time_cfg(None)
+        # the learned tuner's configuration for this shape (its MLP evaluated by
+        # lthip_mlp_forward) against the exhaustive optimum over the same space
+        if w4_model is not None:
+            choice = w4_model.choose([(M, Nw, Kw, G)])[0]
+            space = [c for c in tuner.W4A16_SPACE if tuner.valid(c, M, Kw, Nw, G)]
+            times = {}
+            for c in space:
+                try:
+                    times[c] = time_cfg(W4A16Config(*c), 20.0)
+                except Exception:   # a configuration the library turns down for this shape
+                    pass
+            if choice not in times:
+                times[choice] = float("inf")
+            best = min(times, key=times.get)
+            tuned[f"M{M}"] = {"mlp_config": list(choice), "us": round(times[choice] * 1e3, 2),
+                              "exhaustive_best": list(best), "best_us": round(times[best] * 1e3, 2),
+                              "regret": round(times[choice] / times[best] - 1.0, 4),
+                              "auto_us": round(ms * 1e3, 2), "candidates": len(space)}
+        R.w4a16(At, Wt, St, Zt, G, Ct)
+        del ws
         byts = 2 * M * Kw + Kw // 8 * Nw * 4 + Kw // G * Nw * 2 + Kw // G * Nw // 2 + 2 * M * Nw
         flops = 2 * M * Nw * Kw
         rows = torch.arange(0, M, max(1, M // 64), device=dev)
@@ -362,6 +391,8 @@ def bench_routines(torch, R, timer, dev, quick):
                               "frac_fp16_mfma": round(flops / ms / 1e9 / FP16_PEAK, 4), "max_rel_err_vs_fp64": err,
                               "kernel": w4a16_kind_name(M, Nw, Kw)}
         del Wd, Ar
+    if tuned:
+        out["w4a16_tuned"] = tuned
     # grouped w4a16: many problems, each with its own weights, in one
     # persistent launch; against the same problems as separate automatic
     # launches, both replayed from a hipGraph (copies of the problem set rotate
