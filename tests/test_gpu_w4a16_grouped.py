@@ -183,3 +183,19 @@ def test_grouped_rejects(dev):
                 W4A16Config(3, 256, 128, 2)):
         with pytest.raises(LthipError):
             R.GroupedW4A16(probs, cfg)
+
+
+def test_grouped_many_small_problems(dev, orc):
+    # 300 problems (the table's binary search over many entries, items of all
+    # three tile classes interleaved), each checked against the oracle
+    rng = np.random.default_rng(12)
+    dims = [(int(rng.integers(1, 300)), int(rng.choice([32, 96, 256, 288])), int(rng.choice([64, 128, 256])), 64,
+             bool(rng.integers(0, 2))) for _ in range(300)]
+    host, probs = make(dims, dev, seed=13)
+    plan = R.GroupedW4A16(probs)
+    rows = [256 if M > 128 else (128 if M > 64 else 64) for M, *_ in dims]
+    assert plan.info()["work_items"] == sum(-(-M // r) * -(-N // 256) for (M, N, *_), r in zip(dims, rows))
+    plan.run()
+    torch.cuda.synchronize()
+    check_all(orc, host, probs)
+    plan.close()
