@@ -160,9 +160,9 @@ struct D2 {
 __device__ __forceinline__ int swzA64(int r) { return (r >> 1) & 7; }
 __device__ __forceinline__ int swzB64(int k) { return 8 * (k & 1); }
 
-// the tile as a device function: a kernel template whose own body calls the
-// amdgcn builtins gets no host-side launch handle (the host pass drops the
-// instantiation), so the kernel below only forwards
+// the tile as a device function: with the whole tile in the kernel template's
+// body the host pass emitted no launch handle for its instances (an undefined
+// symbol at link time), so the kernel below only forwards
 template <class D>
 __device__ __forceinline__ void dgemm2_tile(const double* __restrict__ A, const double* __restrict__ B,
                                             double* __restrict__ C, int M, int N, int K, double alpha, double beta,
