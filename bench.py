@@ -498,7 +498,7 @@ def bench_w4a16_grouped(torch, R, inputs, np, timer, dev, it):
 
 def w4a16_kind_name(M, N, K):
     # the automatic choice of lthip_w4a16_gemm for this shape (group 128)
-    if M <= 2 or (M <= 4 and N < 10240):
+    if M <= 2:
         return "decode (kind 4)"
     if M <= 64:
         return "skinny MFMA (kind 5)"

@@ -769,11 +769,13 @@ int tiled2_tail(const Tiled2Entry& e, long M, long N, long K, int* full) {
 void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_w4a16_config* out) {
     lthip_w4a16_config c = cfg ? *cfg : lthip_w4a16_config{0, 0, 0, 0};
     if (c.kind == 0) {
-        if (M <= 2 || (M <= 4 && N < 10240)) {
+        if (M <= 2) {
             // decode on the vector ALUs; 64 columns per block (RG = 4) once
-            // N / 64 fills most CUs, 32 otherwise.  At M = 3, 4 and wide N
-            // the skinny MFMA kernel is faster (measured).  The launch falls
-            // back to kind 5, then kind 1, where the conditions do not hold.
+            // N / 64 fills most CUs, 32 otherwise.  At M = 3, 4 the skinny
+            // MFMA kernel is faster (measured at N = 4096 to 28672: 7.9 vs
+            // 9.8 us at 4096 x 4096, 13.4 vs 15.4 at 8192 x 8192).  The launch
+            // falls back to kind 5, then kind 1, where the conditions do not
+            // hold.
             c.kind = 4;
             c.tile_n = N >= 10240 ? 64 : 32;
             c.tile_m = 0;

@@ -217,8 +217,9 @@ def test_w4a16_decode_exact_and_symmetric(dev, orc):
 
 
 def test_w4a16_decode_auto_matches_stream(dev, orc):
-    # the automatic choice at M <= 2 is the decode kernel; M = 3, 4 at wide N
-    # stay on the MFMA stream kernel; both within the product tolerance
+    # the automatic choice at M <= 2 is the decode kernel, at M = 3, 4 the
+    # skinny MFMA kernel; both within the product tolerance, as is the MFMA
+    # stream kernel
     for M, N in ((1, 11008), (2, 4096), (3, 4096), (4, 10240)):
         K = 1024
         A, q, s, z = inputs.w4a16_problem(M, N, K, 128, seed=7 * M)
