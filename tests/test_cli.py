@@ -41,6 +41,13 @@ def test_missing_arguments_list_implementations():
     assert r.returncode == 1 and "nb_iter" in r.stderr
     r = run("tune", "hgemm", 1, 2)
     assert r.returncode == 1 and "Usage" in r.stderr
+    # the batched routines check their arguments before any HIP call too
+    for args in (("test", "w4a16_grouped"), ("bench", "w4a16_grouped", "bogus"), ("tune", "w4a16_grouped", "graph"),
+                 ("test", "w4a16_grouped", "graph", 4, 520), ("test", "w4a16_grouped", "graph", 0)):
+        r = run(*args)
+        assert r.returncode == 1 and ("Usage" in r.stderr or "need positive" in r.stderr), args
+    r = run("test", "hgemm_batched", "bogus")
+    assert r.returncode == 1 and "Usage" in r.stderr
 
 
 @pytest.mark.gpu
