@@ -369,13 +369,13 @@ def bench_routines(torch, R, timer, dev, quick):
                     times[c] = time_cfg(W4A16Config(*c), 20.0)
                 except Exception:   # a configuration the library turns down for this shape
                     pass
-            if choice not in times:
-                times[choice] = float("inf")
-            best = min(times, key=times.get)
-            tuned[f"M{M}"] = {"mlp_config": list(choice), "us": round(times[choice] * 1e3, 2),
-                              "exhaustive_best": list(best), "best_us": round(times[best] * 1e3, 2),
-                              "regret": round(times[choice] / times[best] - 1.0, 4),
-                              "auto_us": round(ms * 1e3, 2), "candidates": len(space)}
+            best = min(times, key=times.get) if times else None
+            ok = best is not None and choice in times   # (null fields: the library turned the choice down)
+            tuned[f"M{M}"] = {"mlp_config": list(choice), "us": round(times[choice] * 1e3, 2) if ok else None,
+                              "exhaustive_best": list(best) if best else None,
+                              "best_us": round(times[best] * 1e3, 2) if best else None,
+                              "regret": round(times[choice] / times[best] - 1.0, 4) if ok else None,
+                              "auto_us": round(ms * 1e3, 2), "candidates": len(space), "timed": len(times)}
         R.w4a16(At, Wt, St, Zt, G, Ct)
         del ws
         byts = 2 * M * Kw + Kw // 8 * Nw * 4 + Kw // G * Nw * 2 + Kw // G * Nw // 2 + 2 * M * Nw
