@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void grouped_kernel(const GProb* __restrict__ 
     for (;;) {
         if (threadIdx.x == 0) item_s = (int)take_ticket(&queue[0]);
         __syncthreads();
-        const int item = item_s;
+        const int item = __builtin_amdgcn_readfirstlane(item_s);
         __syncthreads();
         if (item >= total) break;
         // problem holding this item: last p with work_begin <= item
@@ -99,7 +99,8 @@ __global__ __launch_bounds__(C::THREADS) void grouped2_kernel(const GProb* __res
     for (;;) {
         if (threadIdx.x == 0) *item_s = (int)take_ticket(&queue[0]);
         __syncthreads();
-        const int item = *item_s;
+        // uniform across the block: the table lookups below become scalar loads
+        const int item = __builtin_amdgcn_readfirstlane(*item_s);
         __syncthreads();
         if (item >= total) break;
         int lo = 0, hi = count - 1;
