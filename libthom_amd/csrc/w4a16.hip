@@ -23,11 +23,15 @@
 // MFMA fragment).  Needs group % 64 == 0, N % 32 == 0 and 16-byte aligned
 // scales/zero-points; the automatic choice falls back to kind 2 otherwise.
 //
-// kind 4, decode (M <= 4): v_dot2 on exactly dequantised (q - z) pairs, the
-// group scale applied once per group; no split-K (described at the kernel).
+// kind 4, decode (M <= 4): v_dot2 on the dequantised weights fp16((q - z) s),
+// each rounded once; no split-K (described at the kernel).
 //
 // kind 5, skinny M (<= 64): the kind-1 lane mapping on MFMA with kind 4's
-// exact (q - z) operands and per-group scaling; no split-K.
+// exactly dequantised weights, 64- or 32-column blocks; no split-K.
+//
+// Every kind forms the reference's dequantised weight fp16((q - z) s) with one
+// rounding before it multiplies (quantize.cpp:58-65); none takes the
+// scale-after-the-sum shortcut of w4a16_naive.cu:12-18.
 #include "w4_core2.hpp"
 
 namespace lt {
@@ -233,19 +237,18 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_stream_kernel(const __half* 
 }
 
 // ---- kind 4: decode (M <= 4) on the vector ALUs -------------------------
-// At M <= 4 the MFMA path spends its time dequantising (a 16x16 MFMA tile
-// wastes 15/16 of its rows) and the weight stream waits on the VALU.  Here
-// one packed uint32 (8 k of one column) costs 1 shift, 4 v_and_or_b32, 4
-// packed fp16 subtractions and 4 v_dot2_f32_f16 per activation row:
+// At M <= 4 a 16x16 MFMA tile wastes 15/16 of its rows; here the products run
+// as v_dot2_f32_f16 on the vector ALUs.  Each weight is first rounded once to
+// fp16((q - z) * s), the reference's dequantised weight (quantize.cpp:58-65),
+// so the kernel sums exactly the products of w4a16.cpp:178-184.  One packed
+// uint32 (8 k of one column) costs 1 shift, 4 v_and_or_b32, 4 packed fp16
+// subtractions, 4 packed fp16 multiplications and 4 v_dot2 per activation row:
 //   (x & 0x000F000F) | 0x64006400 = fp16 pair (1024 + q_k, 1024 + q_k+4)
-//   (x & 0x00F000F0) | 0x64006400 = (1024 + 16 q_k+1, 1024 + 16 q_k+5)
-// and the same on x >> 8 for k+2/k+6, k+3/k+7.  Subtracting 1024 + z (resp.
-// 1024 + 16 z) leaves (q - z), resp. 16 (q - z), exactly; v_dot2 multiplies
-// them with the activation pairs (exact products, FP32 sums) and the group's
-// scale is applied once per group and column: acc += s * (De + Do / 16).
-// The scaled weight is never rounded to fp16, so the result is closer to the
-// exact product than the dequantise-then-multiply order; it stays within the
-// product tolerance of w4a16.cpp:194-197 and is checked against the oracle.
+//   (x & 0x00F000F0) | 0x54005400 = fp16 pair (64 + q_k+1, 64 + q_k+5)
+// (a mantissa field of 16 q under the exponent of 64, whose ulp is 1/16), and
+// the same on x >> 8 for k+2/k+6, k+3/k+7.  Subtracting 1024 + z (resp. 64 + z)
+// leaves (q - z) exactly; the multiplication by the pair (s, s) rounds the
+// exact product once to fp16.
 //
 // Layout: RG weight rows per wave load, 64 / RG lanes per row, 4 columns per
 // lane (one uint4): a block covers 256 / RG columns and all of K.  The
@@ -307,12 +310,13 @@ __global__ __launch_bounds__(kDecodeThreads) void w4a16_decode_kernel(const __ha
         av[i] = *reinterpret_cast<const uint4*>(aval[i] ? A + (long)m * K + 8 * kp : A);
     }
     __builtin_amdgcn_sched_barrier(0);
-    const uint32_t MASK_LO = 0x000F000Fu, MASK_HI = 0x00F000F0u, MAGIC = 0x64006400u;
-    float acc[MR][4];
+    const uint32_t MASK_LO = 0x000F000Fu, MASK_HI = 0x00F000F0u, MAG_LO = 0x64006400u, MAG_HI = 0x54005400u;
+    // two FP32 sums per (row, column): even and odd nibble pairs, added at the end
+    float acc[MR][4], acc2[MR][4];
 #pragma unroll
     for (int m = 0; m < MR; m++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[m][j] = 0.f;
+        for (int j = 0; j < 4; j++) acc[m][j] = acc2[m][j] = 0.f;
     // this wave's groups: wave, wave + WAVES, ... (ngw of them), in batches of GB
     const int ngw = wave < NG ? (NG - 1 - wave) / WAVES + 1 : 0;
     const int nb = (ngw + GB - 1) / GB;
@@ -340,19 +344,17 @@ __global__ __launch_bounds__(kDecodeThreads) void w4a16_decode_kernel(const __ha
             const int g = wave + WAVES * (b * GB + gi);
             if (g > glast) break;
             const uint32_t zs = bt.z[gi] >> zsh;
+            const uint32_t sv[2] = {bt.s[gi].x, bt.s[gi].y};
             uint32_t zze[4], zzo[4];
+            h2v_t ss[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const uint32_t zj = (zs >> (4 * j)) & 0xFu;
-                const uint32_t e = 0x6400u | zj, o = 0x6400u | (zj << 4);
+                const uint32_t e = 0x6400u | zj, o = 0x5400u | (zj << 4);
                 zze[j] = e | (e << 16);
                 zzo[j] = o | (o << 16);
+                ss[j] = as_h2(__builtin_amdgcn_perm(sv[j / 2], sv[j / 2], (j & 1) ? 0x03020302u : 0x01000100u));
             }
-            float de[MR][4], dd[MR][4];
-#pragma unroll
-            for (int m = 0; m < MR; m++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) de[m][j] = dd[m][j] = 0.f;
 #pragma unroll
             for (int qi = 0; qi < QG; qi++) {
                 const int row = (g * QG + qi) * RG + rl;
@@ -364,26 +366,18 @@ __global__ __launch_bounds__(kDecodeThreads) void w4a16_decode_kernel(const __ha
                 for (int j = 0; j < 4; j++) {
                     const uint32_t x = wq[j];
                     const uint32_t t = x >> 8;
-                    const h2v_t e0 = as_h2(and_or(x, MASK_LO, MAGIC)) - as_h2(zze[j]);
-                    const h2v_t e1 = as_h2(and_or(x, MASK_HI, MAGIC)) - as_h2(zzo[j]);
-                    const h2v_t e2 = as_h2(and_or(t, MASK_LO, MAGIC)) - as_h2(zze[j]);
-                    const h2v_t e3 = as_h2(and_or(t, MASK_HI, MAGIC)) - as_h2(zzo[j]);
+                    const h2v_t w0 = (as_h2(and_or(x, MASK_LO, MAG_LO)) - as_h2(zze[j])) * ss[j];
+                    const h2v_t w1 = (as_h2(and_or(x, MASK_HI, MAG_HI)) - as_h2(zzo[j])) * ss[j];
+                    const h2v_t w2 = (as_h2(and_or(t, MASK_LO, MAG_LO)) - as_h2(zze[j])) * ss[j];
+                    const h2v_t w3 = (as_h2(and_or(t, MASK_HI, MAG_HI)) - as_h2(zzo[j])) * ss[j];
 #pragma unroll
                     for (int m = 0; m < MR; m++) {
-                        de[m][j] = __builtin_amdgcn_fdot2(e0, as_h2(a[m].x), de[m][j], false);
-                        dd[m][j] = __builtin_amdgcn_fdot2(e1, as_h2(a[m].y), dd[m][j], false);
-                        de[m][j] = __builtin_amdgcn_fdot2(e2, as_h2(a[m].z), de[m][j], false);
-                        dd[m][j] = __builtin_amdgcn_fdot2(e3, as_h2(a[m].w), dd[m][j], false);
+                        acc[m][j] = __builtin_amdgcn_fdot2(w0, as_h2(a[m].x), acc[m][j], false);
+                        acc2[m][j] = __builtin_amdgcn_fdot2(w1, as_h2(a[m].y), acc2[m][j], false);
+                        acc[m][j] = __builtin_amdgcn_fdot2(w2, as_h2(a[m].z), acc[m][j], false);
+                        acc2[m][j] = __builtin_amdgcn_fdot2(w3, as_h2(a[m].w), acc2[m][j], false);
                     }
                 }
-            }
-            const uint32_t sv[2] = {bt.s[gi].x, bt.s[gi].y};
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const float sj =
-                    __half2float(__ushort_as_half((unsigned short)((j & 1) ? sv[j / 2] >> 16 : sv[j / 2] & 0xFFFFu)));
-#pragma unroll
-                for (int m = 0; m < MR; m++) acc[m][j] = fmaf(sj, fmaf(dd[m][j], 0.0625f, de[m][j]), acc[m][j]);
             }
         }
     };
@@ -404,6 +398,10 @@ __global__ __launch_bounds__(kDecodeThreads) void w4a16_decode_kernel(const __ha
         cur = nxt;
     }
     if (nb > 0) compute(cur, nb - 1);
+#pragma unroll
+    for (int m = 0; m < MR; m++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[m][j] += acc2[m][j];
 #pragma unroll
     for (int off = LPR; off < 64; off <<= 1)
 #pragma unroll
@@ -456,44 +454,42 @@ size_t decode_lds(const DecodeEntry& e, long K) {
     return (size_t)e.mr * (size_t)(K / 8) * 16 + (size_t)kDecodeWaves * e.mr * (4 * (64 / e.rg)) * 4;
 }
 
-// ---- kind 5: skinny M (<= 64) on MFMA with exact (q - z) operands --------
-// The lane mapping of the kind-1 stream kernel (lane l loads columns
-// n0 + 4(l%16)..+3 of weight row 4 ks + l/16; MFMA 16x16x32 j takes columns
-// n0 + 4 r + j as its A operand, 16 activation rows as its B operand), with
-// the decode kernel's arithmetic: each uint32 becomes four fp16 pairs of
-// exact (q - z) (3 shifts, 4 v_and_or_b32, 4 packed subtractions), the MFMAs
-// accumulate one group into fresh FP32 accumulators, and the group's scales
-// (16 consecutive columns per lane, the output layout's) fold them into the
-// totals: acc += s * group_acc.  A block covers 64 columns and all of K; its
-// waves take the groups round robin, batches of UQ k-steps with the next
-// batch's loads in flight; weights, scales and zero-points are nontemporal
-// buffer loads.  Waves are summed in LDS in wave order (deterministic).
-// Output lane l: C[16 mt + l%16][n0 + 16 (l/16) + 4 i + j] = acc[mt][j][i].
-template <int MT, int QS, int WAVES, int UQ>
+// ---- kind 5: skinny M (<= 64) on MFMA with exactly dequantised weights ----
+// C^T = W^T A^T on v_mfma_f32_16x16x32_f16: lane l loads CPL consecutive
+// columns n0 + CPL (l%16) .. of weight row 4 ks + l/16 (one uint4 or uint2),
+// MFMA j takes columns n0 + CPL r + j (r = l%16) as its A operand and 16
+// activation rows as its B operand.  Each uint32 becomes the fragment of
+// fp16((q - z) * s) of its 8 k (kind 4's exact assembly: 1 shift, 4
+// v_and_or_b32, 4 packed subtractions, 4 packed multiplications), so the MFMAs
+// sum exactly the products of the reference's dequantised weights
+// (quantize.cpp:58-65, w4a16.cpp:178-184).  A block covers 16 CPL columns and
+// all of K; its waves take the groups round robin, batches of UQ k-steps with
+// the next batch's loads in flight; weights, scales and zero-points are
+// nontemporal buffer loads.  Waves are summed in LDS in wave order
+// (deterministic).
+// Output lane l: C[16 mt + l%16][n0 + CPL (4 (l/16) + i) + j] = acc[mt][j][i].
+template <int MT, int QS, int WAVES, int UQ, int CPL>
 __global__ __launch_bounds__(64 * WAVES) void w4a16_skinny_kernel(const __half* __restrict__ A, BW4Params p,
                                                                   __half* __restrict__ C, int M, int N, int K) {
     static_assert(UQ % QS == 0, "a batch holds whole groups");
-    constexpr int GB = UQ / QS, AUX = 2;
-    extern __shared__ float sred[];   // [WAVES][MT][4 i][4 j][64]
+    static_assert(CPL == 2 || CPL == 4, "two or four columns per lane");
+    constexpr int GB = UQ / QS, AUX = 2, CT = 16 * CPL;
+    typedef uint32_t wvec __attribute__((ext_vector_type(CPL)));
+    extern __shared__ float sred[];   // [WAVES][MT][4 i][CPL j][64]
     const int KP = K / 8;
     const int NG = KP / (4 * QS);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 15, kb = lane >> 4;
-    const int n0 = blockIdx.x * 64;
-    const int colw = n0 + 4 * r < N ? n0 + 4 * r : N - 4;
-    // scales of columns n0 + 16 kb .. +15 as two 8-column chunks; a chunk that
-    // starts at or past N is never stored, so clamping it keeps loads in range
-    int cs0 = n0 + 16 * kb, cs1 = cs0 + 8;
-    cs0 = cs0 < N - 8 ? cs0 : N - 8;
-    cs1 = cs1 < N - 8 ? cs1 : N - 8;
+    const int n0 = blockIdx.x * CT;
+    const int colw = n0 + CPL * r < N ? n0 + CPL * r : N - CPL;   // clamped: loads in range, results unused
     const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)p.W, 0, KP * N * 4, 0x00020000);
     const __amdgpu_buffer_rsrc_t sr = __builtin_amdgcn_make_buffer_rsrc((void*)p.scales, 0, NG * N * 2, 0x00020000);
     const __amdgpu_buffer_rsrc_t zr =
         __builtin_amdgcn_make_buffer_rsrc((void*)(p.qzeros ? p.qzeros : p.W), 0, NG * (N / 8) * 4, 0x00020000);
     const __amdgpu_buffer_rsrc_t ar = __builtin_amdgcn_make_buffer_rsrc((void*)A, 0, M * K * 2, 0x00020000);
     const bool has_z = p.qzeros != nullptr;
-    const int wv_off = (kb * N + colw) * 4, zv_off = (colw / 8) * 4;
+    const int wv_off = (kb * N + colw) * 4, sv_off = colw * 2, zv_off = (colw / 8) * 4;
     const int zsh = 4 * (colw & 7);
     int av_off[MT];
     bool mv[MT];
@@ -503,19 +499,19 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_skinny_kernel(const __half* 
         mv[mt] = m < M;
         av_off[mt] = ((mv[mt] ? m : M - 1) * K + 8 * kb) * 2;
     }
-    const uint32_t MASK = 0x000F000Fu, MAGIC = 0x64006400u;
-    v4f acc[MT][4];
+    const uint32_t MASK_LO = 0x000F000Fu, MASK_HI = 0x00F000F0u, MAG_LO = 0x64006400u, MAG_HI = 0x54005400u;
+    v4f acc[MT][CPL];
 #pragma unroll
     for (int mt = 0; mt < MT; mt++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[mt][j] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < CPL; j++) acc[mt][j] = v4f{0.f, 0.f, 0.f, 0.f};
     const int ngw = wave < NG ? (NG - 1 - wave) / WAVES + 1 : 0;
     const int nb = (ngw + GB - 1) / GB;
     const int glast = wave + WAVES * (ngw - 1);
     struct Batch {
-        u4v_t w[UQ];
+        wvec w[UQ];
         u4v_t a[UQ][MT];
-        u4v_t s0[GB], s1[GB];
+        uint32_t s[GB][CPL / 2];
         uint32_t z[GB];
     };
     auto load = [&](Batch& bt, int b) {
@@ -524,16 +520,27 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_skinny_kernel(const __half* 
             int g = wave + WAVES * (b * GB + gi);
             g = g <= glast ? g : glast;   // past the wave's last group: a repeat, not used
             bt.z[gi] = has_z ? __builtin_amdgcn_raw_buffer_load_b32(zr, zv_off, g * (N / 8) * 4, AUX) : 0x88888888u;
+            if constexpr (CPL == 4) {
+                const u2v_t sv = __builtin_amdgcn_raw_buffer_load_b64(sr, sv_off, g * N * 2, AUX);
+                bt.s[gi][0] = sv.x;
+                bt.s[gi][1] = sv.y;
+            } else {
+                bt.s[gi][0] = __builtin_amdgcn_raw_buffer_load_b32(sr, sv_off, g * N * 2, AUX);
+            }
 #pragma unroll
             for (int qi = 0; qi < QS; qi++) {
                 const int ks = g * QS + qi;
-                bt.w[gi * QS + qi] = __builtin_amdgcn_raw_buffer_load_b128(wr, wv_off, ks * 4 * N * 4, AUX);
+                if constexpr (CPL == 4) {
+                    const u4v_t wq = __builtin_amdgcn_raw_buffer_load_b128(wr, wv_off, ks * 4 * N * 4, AUX);
+                    bt.w[gi * QS + qi] = wvec{wq.x, wq.y, wq.z, wq.w};
+                } else {
+                    const u2v_t wq = __builtin_amdgcn_raw_buffer_load_b64(wr, wv_off, ks * 4 * N * 4, AUX);
+                    bt.w[gi * QS + qi] = wvec{wq.x, wq.y};
+                }
 #pragma unroll
                 for (int mt = 0; mt < MT; mt++)
                     bt.a[gi * QS + qi][mt] = __builtin_amdgcn_raw_buffer_load_b128(ar, av_off[mt], ks * 64, 0);
             }
-            bt.s0[gi] = __builtin_amdgcn_raw_buffer_load_b128(sr, cs0 * 2, g * N * 2, AUX);
-            bt.s1[gi] = __builtin_amdgcn_raw_buffer_load_b128(sr, cs1 * 2, g * N * 2, AUX);
         }
     };
     auto compute = [&](const Batch& bt, int b) {
@@ -542,17 +549,17 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_skinny_kernel(const __half* 
             const int g = wave + WAVES * (b * GB + gi);
             if (g > glast) break;
             const uint32_t zs = bt.z[gi] >> zsh;
-            uint32_t zz[4];
+            uint32_t zze[CPL], zzo[CPL];
+            h2v_t ss[CPL];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t e = 0x6400u | ((zs >> (4 * j)) & 0xFu);
-                zz[j] = e | (e << 16);
+            for (int j = 0; j < CPL; j++) {
+                const uint32_t zj = (zs >> (4 * j)) & 0xFu;
+                const uint32_t e = 0x6400u | zj, o = 0x5400u | (zj << 4);
+                zze[j] = e | (e << 16);
+                zzo[j] = o | (o << 16);
+                const uint32_t sw = bt.s[gi][j / 2];
+                ss[j] = as_h2(__builtin_amdgcn_perm(sw, sw, (j & 1) ? 0x03020302u : 0x01000100u));
             }
-            v4f ag[MT][4];
-#pragma unroll
-            for (int mt = 0; mt < MT; mt++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) ag[mt][j] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int qi = 0; qi < QS; qi++) {
                 v8h af[MT];
@@ -563,15 +570,16 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_skinny_kernel(const __half* 
                     if (!mv[mt]) pa = make_uint4(0, 0, 0, 0);
                     __builtin_memcpy(&af[mt], &pa, 16);
                 }
-                const u4v_t wq = bt.w[gi * QS + qi];
+                const wvec wq = bt.w[gi * QS + qi];
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
+                for (int j = 0; j < CPL; j++) {
                     const uint32_t x = wq[j];
-                    // fp16 pairs (k, k + 4) of exact (q - z), in the perm_k8 order
-                    const h2v_t e0 = as_h2(and_or(x, MASK, MAGIC)) - as_h2(zz[j]);
-                    const h2v_t e1 = as_h2(and_or(x >> 4, MASK, MAGIC)) - as_h2(zz[j]);
-                    const h2v_t e2 = as_h2(and_or(x >> 8, MASK, MAGIC)) - as_h2(zz[j]);
-                    const h2v_t e3 = as_h2(and_or(x >> 12, MASK, MAGIC)) - as_h2(zz[j]);
+                    const uint32_t t = x >> 8;
+                    // fp16 pairs (k, k + 4) of fp16((q - z) s), in the perm_k8 order
+                    const h2v_t e0 = (as_h2(and_or(x, MASK_LO, MAG_LO)) - as_h2(zze[j])) * ss[j];
+                    const h2v_t e1 = (as_h2(and_or(x, MASK_HI, MAG_HI)) - as_h2(zzo[j])) * ss[j];
+                    const h2v_t e2 = (as_h2(and_or(t, MASK_LO, MAG_LO)) - as_h2(zze[j])) * ss[j];
+                    const h2v_t e3 = (as_h2(and_or(t, MASK_HI, MAG_HI)) - as_h2(zzo[j])) * ss[j];
                     v8h wf;
                     wf[0] = e0[0];
                     wf[1] = e0[1];
@@ -583,21 +591,9 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_skinny_kernel(const __half* 
                     wf[7] = e3[1];
 #pragma unroll
                     for (int mt = 0; mt < MT; mt++)
-                        ag[mt][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wf, af[mt], ag[mt][j], 0, 0, 0);
+                        acc[mt][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wf, af[mt], acc[mt][j], 0, 0, 0);
                 }
             }
-            const uint32_t sv[8] = {bt.s0[gi].x, bt.s0[gi].y, bt.s0[gi].z, bt.s0[gi].w,
-                                    bt.s1[gi].x, bt.s1[gi].y, bt.s1[gi].z, bt.s1[gi].w};
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int c = 4 * i + j;   // the scale of column n0 + 16 kb + c
-                    const float sc = __half2float(
-                        __ushort_as_half((unsigned short)((c & 1) ? sv[c / 2] >> 16 : sv[c / 2] & 0xFFFFu)));
-#pragma unroll
-                    for (int mt = 0; mt < MT; mt++) acc[mt][j][i] = fmaf(sc, ag[mt][j][i], acc[mt][j][i]);
-                }
         }
     };
     Batch cur;
@@ -614,45 +610,56 @@ __global__ __launch_bounds__(64 * WAVES) void w4a16_skinny_kernel(const __half* 
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) sred[(((wave * MT + mt) * 4 + i) * 4 + j) * 64 + lane] = acc[mt][j][i];
+            for (int j = 0; j < CPL; j++) sred[(((wave * MT + mt) * 4 + i) * CPL + j) * 64 + lane] = acc[mt][j][i];
     __syncthreads();
-    // thread (mt, i, lane): four consecutive columns, waves summed in order
+    // thread (mt, i, lane): CPL consecutive columns, waves summed in order
     for (int t = threadIdx.x; t < MT * 4 * 64; t += 64 * WAVES) {
         const int l = t & 63, i = (t >> 6) & 3, mt = t >> 8;
-        const int m = 16 * mt + (l & 15), n = n0 + 16 * (l >> 4) + 4 * i;
-        float v[4] = {0.f, 0.f, 0.f, 0.f};
+        const int m = 16 * mt + (l & 15), n = n0 + CPL * (4 * (l >> 4) + i);
+        float v[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; j++) v[j] = 0.f;
         for (int w2 = 0; w2 < WAVES; w2++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) v[j] += sred[(((w2 * MT + mt) * 4 + i) * 4 + j) * 64 + l];
+            for (int j = 0; j < CPL; j++) v[j] += sred[(((w2 * MT + mt) * 4 + i) * CPL + j) * 64 + l];
         if (m < M && n < N) {
-            const uint32_t lo = (uint32_t)__half_as_ushort(__float2half_rn(v[0])) |
-                                ((uint32_t)__half_as_ushort(__float2half_rn(v[1])) << 16);
-            const uint32_t hi = (uint32_t)__half_as_ushort(__float2half_rn(v[2])) |
-                                ((uint32_t)__half_as_ushort(__float2half_rn(v[3])) << 16);
-            *reinterpret_cast<uint2*>(C + (long)m * N + n) = make_uint2(lo, hi);
+            uint32_t pk[CPL / 2];
+#pragma unroll
+            for (int h = 0; h < CPL / 2; h++)
+                pk[h] = (uint32_t)__half_as_ushort(__float2half_rn(v[2 * h])) |
+                        ((uint32_t)__half_as_ushort(__float2half_rn(v[2 * h + 1])) << 16);
+            if constexpr (CPL == 4)
+                *reinterpret_cast<uint2*>(C + (long)m * N + n) = make_uint2(pk[0], pk[1]);
+            else
+                *reinterpret_cast<uint32_t*>(C + (long)m * N + n) = pk[0];
         }
     }
 }
 
 struct SkinnyEntry {
-    int mt, qs, waves;
+    int mt, qs, waves, cpl;
     decode_fn fn;
 };
-#define LT_W4K(mt, qs, w) {mt, qs, w, w4a16_skinny_kernel<mt, qs, w, (qs > 4 ? qs : 4)>},
-#define LT_W4K_QS(mt, w) LT_W4K(mt, 1, w) LT_W4K(mt, 2, w) LT_W4K(mt, 4, w) LT_W4K(mt, 8, w)
-// one row tile: 16 waves where the batch fits 128 VGPRs (group 128), else 8
-const SkinnyEntry kSkinny[] = {LT_W4K(1, 1, 8) LT_W4K(1, 2, 8) LT_W4K(1, 4, 16) LT_W4K(1, 8, 8) LT_W4K_QS(2, 4)
-                                   LT_W4K_QS(4, 4)};
+#define LT_W4K(mt, qs, w, cpl) {mt, qs, w, cpl, w4a16_skinny_kernel<mt, qs, w, (qs > 4 ? qs : 4), cpl>},
+#define LT_W4K_QS(mt, w, cpl) LT_W4K(mt, 1, w, cpl) LT_W4K(mt, 2, w, cpl) LT_W4K(mt, 4, w, cpl) LT_W4K(mt, 8, w, cpl)
+// 64-column blocks (CPL 4) and 32-column blocks (CPL 2): one row tile with 16
+// waves where the batch fits 128 VGPRs (group 128), else 8
+const SkinnyEntry kSkinny[] = {LT_W4K(1, 1, 8, 4) LT_W4K(1, 2, 8, 4) LT_W4K(1, 4, 16, 4) LT_W4K(1, 8, 8, 4)
+                                   LT_W4K_QS(2, 4, 4) LT_W4K_QS(4, 4, 4)
+                                   LT_W4K(1, 1, 8, 2) LT_W4K(1, 2, 8, 2) LT_W4K(1, 4, 16, 2) LT_W4K(1, 8, 8, 2)
+                                   LT_W4K_QS(2, 4, 2) LT_W4K_QS(4, 4, 2)};
 #undef LT_W4K_QS
 #undef LT_W4K
 
-// Kind-5 launch entry for M x N x K (group), or null outside its domain.
-const SkinnyEntry* skinny_entry(long M, long N, long K, int group) {
+// Kind-5 launch entry for M x N x K (group) and 64- or 32-column blocks, or
+// null outside its domain.
+const SkinnyEntry* skinny_entry(long M, long N, long K, int group, int tile_n) {
+    if (tile_n != 64 && tile_n != 32) return nullptr;
     if (M < 1 || M > 64 || K <= 0 || N < 8 || group % 32 != 0) return nullptr;
     if ((K / 8) * N * 4 >= (1L << 31) || M * K * 2 >= (1L << 31)) return nullptr;   // 32-bit buffer offsets
-    const int mt = M <= 16 ? 1 : (M <= 32 ? 2 : 4), qs = group / 32;
+    const int mt = M <= 16 ? 1 : (M <= 32 ? 2 : 4), qs = group / 32, cpl = tile_n / 16;
     for (const SkinnyEntry& e : kSkinny)
-        if (e.mt == mt && e.qs == qs) return &e;
+        if (e.mt == mt && e.qs == qs && e.cpl == cpl) return &e;
     return nullptr;
 }
 
@@ -880,12 +887,13 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
     }
     if (c.kind == 5) {
         const bool explicit5 = cfg && cfg->kind == 5;
-        if (explicit5 && (c.split_k != 1 || c.tile_n != 64))
-            return fail(LTHIP_ERR_INVALID_ARG, "w4a16: kind 5 takes tile_n = 64 and split_k = 1");
-        const SkinnyEntry* e = skinny_entry((long)M, (long)N, (long)K, group);
+        if (explicit5 && (c.split_k != 1 || (c.tile_n != 64 && c.tile_n != 32)))
+            return fail(LTHIP_ERR_INVALID_ARG, "w4a16: kind 5 takes tile_n = 64 or 32 and split_k = 1");
+        const SkinnyEntry* e = skinny_entry((long)M, (long)N, (long)K, group, c.tile_n);
         if (e) {
-            const size_t lds = (size_t)e->waves * e->mt * 16 * 64 * 4;
-            hipLaunchKernelGGL(e->fn, dim3((unsigned)((n + 63) / 64)), dim3((unsigned)(64 * e->waves)), lds, s,
+            const int ct = 16 * e->cpl;
+            const size_t lds = (size_t)e->waves * e->mt * 4 * e->cpl * 64 * 4;
+            hipLaunchKernelGGL(e->fn, dim3((unsigned)((n + ct - 1) / ct)), dim3((unsigned)(64 * e->waves)), lds, s,
                                reinterpret_cast<const __half*>(A), p, reinterpret_cast<__half*>(C), m, n, k);
             LT_TRY_LAUNCH("w4a16 skinny launch");
             return LTHIP_OK;

@@ -247,24 +247,27 @@ def test_w4a16_decode_rejects_unsupported(dev):
 
 
 KIND5 = W4A16Config(5, 64, 0, 1)
+KIND5_32 = W4A16Config(5, 32, 0, 1)
 
 
+@pytest.mark.parametrize("cfg", [KIND5, KIND5_32], ids=["cols64", "cols32"])
 @pytest.mark.parametrize("M", [1, 5, 16, 17, 33, 64])
 @pytest.mark.parametrize("group", [32, 64, 128, 256])
-def test_w4a16_skinny(dev, orc, M, group):
+def test_w4a16_skinny(dev, orc, cfg, M, group):
     N, K = 520, 1024
     A, q, s, z = inputs.w4a16_problem(M, N, K, group, seed=171 + M)
     packed, qz = inputs.pack_w4(q), inputs.pack_zeros(z)
-    C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), group, config=KIND5).cpu().numpy()
+    C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), group, config=cfg).cpu().numpy()
     w4_close(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, group)))
 
 
+@pytest.mark.parametrize("cfg", [KIND5, KIND5_32], ids=["cols64", "cols32"])
 @pytest.mark.parametrize("N,K", [(8, 128), (24, 256), (4104, 384)])
-def test_w4a16_skinny_edges(dev, orc, N, K):
+def test_w4a16_skinny_edges(dev, orc, cfg, N, K):
     M = 20
     A, q, s, z = inputs.w4a16_problem(M, N, K, 128, seed=N + 3 * K)
     packed, qz = inputs.pack_w4(q), inputs.pack_zeros(z)
-    C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), 128, config=KIND5).cpu().numpy()
+    C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), 128, config=cfg).cpu().numpy()
     w4_close(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, 128)))
 
 
@@ -289,7 +292,7 @@ def test_w4a16_skinny_rejects_unsupported(dev):
     A, q, s, z = inputs.w4a16_problem(65, 256, 512, 128, seed=19)
     packed, qz = i32(inputs.pack_w4(q), dev), i32(inputs.pack_zeros(z), dev)
     At, st = t(A, dev), t(s, dev)
-    for M, cfg in ((65, KIND5), (16, W4A16Config(5, 32, 0, 1)), (16, W4A16Config(5, 64, 0, 2))):
+    for M, cfg in ((65, KIND5), (16, W4A16Config(5, 48, 0, 1)), (16, W4A16Config(5, 64, 0, 2))):
         with pytest.raises(LthipError):
             R.w4a16(At[:M], packed, st, qz, 128, config=cfg)
     A2, q2, s2, z2 = inputs.w4a16_problem(8, 64, 512, 16, seed=20)   # group 16

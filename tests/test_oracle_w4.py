@@ -156,3 +156,19 @@ def test_quantize_asym_round_half_even(orc):
     q, z, s = orc.w4_quantize_asym(W, 8)
     assert float(s[0, 0]) == 1.0 and int(z[0, 0]) == 7
     assert list(q[:, 0]) == [0, 15, 7, 9, 9, 7, 5, 7]
+
+
+def test_exact_inputs_expose_shortcut(orc):
+    # the bit-exact GPU test's inputs: the oracle equals an exact float64
+    # evaluation of sum A * fp16((q - z) s), and the scale-after-the-sum order
+    # of w4a16_naive.cu:12-18 differs from it in many outputs
+    from _w4_exact import exact_problem, shortcut
+    for M, group in ((1, 32), (16, 32), (64, 64)):
+        A, q, z, s = exact_problem(M, 288, 64, group, seed=1000 + 7 * M + group)
+        packed, qz = orc.w4_pack(q), orc.w4_pack_zeros(z)
+        W = orc.w4_dequant(packed, s, qz, group)
+        want = orc.w4a16_rows(A, W)
+        exact = (A.astype(np.float64) @ W.astype(np.float64)).astype(np.float32).astype(np.float16)
+        assert np.array_equal(want.view(np.uint16), exact.view(np.uint16))
+        differ = (shortcut(A, q, z, s, group).view(np.uint16) != want.view(np.uint16)).mean()
+        assert differ > 0.1, differ
