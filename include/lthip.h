@@ -81,6 +81,16 @@ int lthip_probe_read_f32(const float *x, size_t n, float *sink, int blocks, void
 size_t lthip_reduce_sum_f32_workspace_size(size_t n);
 int lthip_reduce_sum_f32(const float *x, size_t n, float *out, void *ws, size_t ws_bytes,
                          void *stream);
+/* ---- allocation-free reduction: no counterpart in the reference.  For callers
+ *      that cannot allocate (code under HIP graph capture, tight serving loops,
+ *      harnesses that forbid hipMalloc): the same kernel and result as
+ *      lthip_reduce_sum_f32, with the arrival counter and block partials in a
+ *      static per-GPU scratch array of 16,640 bytes (held until the process
+ *      ends) instead of a caller workspace.  Any n fits (the grid is capped at
+ *      4096 blocks); a launch that would need more scratch is rejected with
+ *      LTHIP_ERR_WORKSPACE.  Calls no allocator.  All calls on one GPU share the
+ *      array: calls that overlap on one GPU (different streams or threads) race. */
+int lthip_reduce_sum_f32_noalloc(const float *x, size_t n, float *out, void *stream);
 
 /* ---- sdot: replaces SdotImpl float(*)(int, const float*, const float*, float*)
  *      (sdot.cpp:32; sdot_warp.cu:101).  *result = sum x[i]*y[i] in FP32. -- */

@@ -70,6 +70,7 @@ _SIGS = {
     "lthip_probe_read_f32": (c_int, [c_vp, c_size, c_vp, c_int, c_vp]),
     "lthip_reduce_sum_f32_workspace_size": (c_size, [c_size]),
     "lthip_reduce_sum_f32": (c_int, [c_vp, c_size, c_vp, c_vp, c_size, c_vp]),
+    "lthip_reduce_sum_f32_noalloc": (c_int, [c_vp, c_size, c_vp, c_vp]),
     "lthip_sdot_f32_workspace_size": (c_size, [c_size]),
     "lthip_sdot_f32": (c_int, [c_size, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "lthip_sgemv_f32_workspace_size": (c_size, [c_int, c_int, c_int]),

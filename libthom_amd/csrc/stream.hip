@@ -8,6 +8,8 @@
 // (reduction_warp.cu:94-105, sdot_warp.cu:117-136).
 #include "lt_common.hpp"
 
+#include <mutex>
+
 namespace lt {
 namespace {
 
@@ -179,6 +181,9 @@ int launch_sum(const float* x, const float* y, size_t n, float* out, void* ws, s
     unsigned* counter = reinterpret_cast<unsigned*>(ws);
     float* partials = reinterpret_cast<float*>(reinterpret_cast<char*>(ws) + 256);
     const unsigned blocks = sum_blocks(n, DOT);
+    if (256 + sizeof(float) * (size_t)blocks > ws_bytes)   // the counter head and one partial per block
+        return fail(LTHIP_ERR_WORKSPACE, "%u blocks need %zu workspace bytes, %zu given", blocks,
+                    256 + sizeof(float) * (size_t)blocks, ws_bytes);
     const bool vec = (reinterpret_cast<uintptr_t>(x) % 16 == 0) && (!DOT || reinterpret_cast<uintptr_t>(y) % 16 == 0);
     if (vec)
         hipLaunchKernelGGL((sum_kernel<DOT, true>), dim3(blocks), dim3(kRBlock), 0, as_stream(stream), x, y, n, out,
@@ -187,6 +192,43 @@ int launch_sum(const float* x, const float* y, size_t n, float* out, void* ws, s
         hipLaunchKernelGGL((sum_kernel<DOT, false>), dim3(blocks), dim3(kRBlock), 0, as_stream(stream), x, y, n, out,
                            counter, partials);
     LT_TRY_LAUNCH(DOT ? "sdot launch" : "reduction launch");
+    return LTHIP_OK;
+}
+
+// ---- allocation-free reduction -------------------------------------------
+// The arrival counter (first 256 bytes) and the block partials of sum_kernel
+// for callers that cannot allocate.  Each GPU holds its own copy at its own
+// address, found once per device with hipGetSymbolAddress; host code uses that
+// address, never this array's name.  Static storage starts zeroed, the first
+// call on a device clears the counter head again, and every launch leaves the
+// counter at zero (the last block resets it).  All calls on one GPU share the
+// array: calls that overlap on one GPU race.
+constexpr size_t kScratchBytes = kRWorkspace;   // 256 + 4 * 4096 = 16,640 bytes per GPU
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+
+__global__ void scratch_head_clear_kernel(unsigned* head) { head[threadIdx.x] = 0u; }
+
+// This device's scratch address (set up on the first call with the device current).
+int scratch_on_device(void** out, void* stream) {
+    static std::mutex mu;
+    static void* ptrs[kMaxDevices] = {nullptr};
+    int dev = 0;
+    int st = check_hip(hipGetDevice(&dev), "reduce_sum_noalloc: hipGetDevice");
+    if (st != LTHIP_OK) return st;
+    if (dev < 0 || dev >= kMaxDevices) return fail(LTHIP_ERR_UNSUPPORTED, "reduce_sum_noalloc: device id %d", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        st = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "reduce_sum_noalloc: hipGetSymbolAddress");
+        if (st != LTHIP_OK) return st;
+        // a small kernel, not hipMemsetAsync, so the clear is safe under stream capture
+        hipLaunchKernelGGL(scratch_head_clear_kernel, dim3(1), dim3(64), 0, as_stream(stream),
+                           reinterpret_cast<unsigned*>(p));
+        LT_TRY_LAUNCH("reduce_sum_noalloc: scratch clear");
+        ptrs[dev] = p;
+    }
+    *out = ptrs[dev];
     return LTHIP_OK;
 }
 
@@ -235,6 +277,13 @@ int lthip_reduce_sum_f32(const float* x, size_t n, float* out, void* ws, size_t 
 int lthip_sdot_f32(size_t n, const float* x, const float* y, float* result, void* ws, size_t ws_bytes,
                    void* stream) {
     return lt::launch_sum<true>(x, y, n, result, ws, ws_bytes, stream);
+}
+
+int lthip_reduce_sum_f32_noalloc(const float* x, size_t n, float* out, void* stream) {
+    void* s = nullptr;
+    const int st = lt::scratch_on_device(&s, stream);
+    if (st != LTHIP_OK) return st;
+    return lt::launch_sum<false>(x, nullptr, n, out, s, lt::kScratchBytes, stream);
 }
 
 }  // extern "C"
