@@ -6,7 +6,8 @@ matrix.  `value` is the achieved fraction of the 8 TB/s HBM peak, in percent;
 `roofline` carries the absolute GB/s.  The `routines` object adds the other
 routines at their configured shapes (reduction n=1e6 and 5e8, sdot, native
 dgemm 4096^3, hgemm 4096^3, w4a16 M in {1,16,128,2048}).  `cpu_baseline`
-times the oracle's sgemv on the host (one thread).  `measured_peaks` holds
+times the oracle's sgemv on the host (all usable cores, up to 16, and one
+thread), with the core count and CPU model.  `measured_peaks` holds
 what this device delivers (MFMA probe loops, a read-only HBM stream and a
 copy); every routine carries its fraction of the datasheet peak and
 `frac_measured`, its fraction of the measured one, and `max_rel_err_vs_fp64`,
@@ -14,6 +15,8 @@ its largest error against an FP64 evaluation of the same inputs (divided by
 the sum of absolute terms, |A||B| or |A||x|; reduction: by the sum).
 `vendor_reference` times
 torch.matmul (hipBLASLt / rocBLAS) on the GEMM shapes, for context only.
+`sizes_sweeps` holds the reference's own benchmark sweeps (each routine's
+SIZES through `libthom bench <routine> hip`, read back from its CSVs).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -40,6 +43,36 @@ SG_BYTES = 4 * (M_SG * N_SG + N_SG + M_SG)
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _orc():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle
+    return _oracle
+
+
+def dot2_err(got, A, B, rows, cols):
+    """Largest |got_i - (A B)[rows_i, cols_i]| / (|A||B|)[rows_i, cols_i] over the
+    sampled entries, the exact product taken from the oracle's compensated dot
+    product (Dot2, twice the FP64 working precision) on the host.  A, B are
+    host arrays (converted to float64), got the device results at the entries."""
+    import numpy as np
+    orc = _orc()
+    r = np.unique(np.asarray(rows, np.int64))
+    c = np.unique(np.asarray(cols, np.int64))
+    Ar = np.asarray(A[r], dtype=np.float64)
+    Bc = np.asarray(B[:, c], dtype=np.float64)
+    ri = np.searchsorted(r, rows).astype(np.int32)
+    ci = np.searchsorted(c, cols).astype(np.int32)
+    hi, lo = orc.dgemm_dot2(Ar, Bc, ri, ci)
+    scale = np.einsum("ik,ki->i", np.abs(Ar[ri]), np.abs(Bc[:, ci]))
+    return float(np.max(np.abs(np.asarray(got, np.float64) - (hi + lo)) / scale))
+
+
+def sample_entries(m, n, count, seed=0):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, m, count), rng.integers(0, n, count)
 
 
 class Timer:
@@ -101,25 +134,115 @@ def traffic_from_profiles(kernel_key):
         return None
 
 
+# bytes per element/problem of each routine, for GB/s next to the CLI's GFLOP/s
+def _sweep_rate(routine, sizes, gflops):
+    if routine in ("reduction", "sdot"):
+        n = sizes[0]
+        flop = (n - 1) if routine == "reduction" else 2 * n
+        byts = (4 if routine == "reduction" else 8) * n
+        return {"GB/s": round(gflops * byts / flop, 1)}
+    if routine == "sgemv":
+        m, n = sizes
+        return {"GB/s": round(gflops * 4 * (m * n + m + n) / (2 * m * n), 1)}
+    return {"TFLOP/s": round(gflops / 1e3, 2)}
+
+
+def cli_sizes_sweeps(timeout_s=240):
+    """The reference's own benchmark sweeps: `libthom bench <routine> hip` runs
+    each routine's SIZES (reduction.cpp:9, sdot.cpp:19, sgemv.cpp:18, dgemm.cpp:17,
+    hgemm.cu:17-27, w4a16.cpp:17-27) in the reference's bench mode (BENCH_RUNS
+    timed runs after WARMUP_RUNS, caches flushed per run) and writes
+    bench_<routine>_hip.csv (routines.h:157-158); the rows are read back here."""
+    import csv
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "libthom_amd", "bin", "libthom")
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for routine in ("reduction", "sdot", "sgemv", "dgemm", "hgemm", "w4a16"):
+            log(f"libthom bench {routine} hip")
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, "bench", routine, "hip"], cwd=tmp, capture_output=True, text=True,
+                               timeout=timeout_s)
+            if r.returncode != 0:
+                raise RuntimeError(f"libthom bench {routine} hip failed: {r.stdout[-800:]} {r.stderr[-800:]}")
+            rows = []
+            with open(os.path.join(tmp, f"bench_{routine}_hip.csv")) as f:
+                for row in csv.DictReader(f):
+                    dims = [int(row[k]) for k in ("size",) if k in row] or \
+                           [int(row[k]) for k in ("M", "N", "K") if k in row]
+                    g = float(row["gflops"])
+                    e = {"size": dims, "GFLOP/s": round(g, 2), **_sweep_rate(routine, dims, g)}
+                    if routine == "w4a16":
+                        e.update({"us": float(row["time_us"]), "GB/s": float(row["gbps"]),
+                                  "dense_fp16_us": float(row["cublas_time_us"]),
+                                  "speedup_vs_dense_fp16": float(row["speedup_vs_cublas"])})
+                    rows.append(e)
+            out[routine] = {"rows": rows, "seconds": round(time.perf_counter() - t0, 1)}
+    return out
+
+
+def host_cpu():
+    """(CPU model string, CPUs this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return model, n
+
+
 def cpu_baseline(seconds=10.0):
-    """The oracle's sgemv (sgemv.cpp:132-147 restated, one thread) on the host."""
+    """The oracle's sgemv (sgemv.cpp:132-147 restated: FP64 dot per row) on the
+    host, on all the cores this process may use (row blocks, one per thread;
+    the oracle call releases the GIL) and on one thread."""
     import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as orc
     from libthom_amd import inputs
     A = inputs.uniform_f32(M_SG * N_SG, -1.0, 1.0, 1234, inputs.S_SGEMV_A)
     x = inputs.uniform_f32(N_SG, -1.0, 1.0, 1234, inputs.S_SGEMV_X)
     y = np.zeros(M_SG, np.float32)
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        orc.sgemv(True, M_SG, N_SG, 1.0, A, N_SG, x, 1, 0.0, y, 1)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    gbs = SG_BYTES * reps / el / 1e9
-    return {"value": round(gbs, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"full 16384x8192 sgemv, {reps} runs in {el:.1f} s (oracle/oracle.c orc_sgemv, FP64 dot per row)",
+    model, ncpu = host_cpu()
+    threads = max(1, min(16, ncpu))   # a GPU box's share is 16 CPUs per GPU
+
+    def timed(fn, budget):
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            fn()
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= budget:
+                return reps, el
+
+    bounds = [M_SG * t // threads for t in range(threads + 1)]
+
+    def block(t):
+        r0, r1 = bounds[t], bounds[t + 1]
+        return orc.sgemv(True, r1 - r0, N_SG, 1.0, A[r0 * N_SG:r1 * N_SG], N_SG, x, 1, 0.0, y[r0:r1], 1)
+
+    with ThreadPoolExecutor(threads) as pool:
+        full = np.concatenate(list(pool.map(block, range(threads))))
+        assert np.array_equal(full, orc.sgemv(True, M_SG, N_SG, 1.0, A, N_SG, x, 1, 0.0, y, 1))
+        reps_mt, el_mt = timed(lambda: list(pool.map(block, range(threads))), seconds / 2)
+    reps_1, el_1 = timed(lambda: orc.sgemv(True, M_SG, N_SG, 1.0, A, N_SG, x, 1, 0.0, y, 1), seconds / 2)
+    gbs_mt = SG_BYTES * reps_mt / el_mt / 1e9
+    gbs_1 = SG_BYTES * reps_1 / el_1 / 1e9
+    return {"value": round(gbs_mt, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"full 16384x8192 sgemv, {reps_mt} runs in {el_mt:.1f} s on {threads} threads "
+                      f"(oracle/oracle.c orc_sgemv on row blocks, FP64 dot per row)",
+            "single_thread": {"value": round(gbs_1, 3), "unit": "GB/s", "cores": 1,
+                              "sample": f"full sgemv, {reps_1} runs in {el_1:.1f} s"},
+            "host_cpus": ncpu, "cpu_model": model,
             "per_routine": cpu_oracle_routines()}
 
 
@@ -204,11 +327,11 @@ def bench_routines(torch, R, timer, dev, quick):
     col()
     ms = timer.per_launch_ms(col, it)
     gbs = 4 * (m * n + n + m) / ms / 1e6
-    Ad, xd = Ac.double(), xc.double()
-    err = ((yc.double() - Ad.t() @ xd).abs() / (Ad.abs().t() @ xd.abs())).max().item()
+    rows, _ = sample_entries(m, 1, 2048)
+    err = dot2_err(yc.cpu().numpy()[rows], Ac.cpu().numpy().T, xc.cpu().numpy()[:, None], rows, 0 * rows)
     out["sgemv_col_16384x8192"] = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
                                    "max_rel_err_vs_fp64": err}
-    del Ac, Ad, xd
+    del Ac
     torch.cuda.empty_cache()
     # reduction
     for n in (1_000_000, 500_000_000):
@@ -218,7 +341,7 @@ def bench_routines(torch, R, timer, dev, quick):
         R.reduce_sum(x, o)
         ms = timer.per_launch_ms(lambda: R.reduce_sum(x, o), it, flush if n < 100_000_000 else None)
         gbs = 4 * n / ms / 1e6
-        ref = x.double().sum().item()
+        ref = _orc().sum_f64(x.cpu().numpy())   # the oracle's FP64 sum
         out[f"reduction_n{n}"] = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
                                   "GFLOP/s": round((n - 1) / ms / 1e6, 1),
                                   "max_rel_err_vs_fp64": abs(o.item() - ref) / abs(ref)}
@@ -233,11 +356,11 @@ def bench_routines(torch, R, timer, dev, quick):
         R.sdot(xs, ys, o)
         ms = timer.per_launch_ms(lambda: R.sdot(xs, ys, o), it, flush if n < 100_000_000 else None)
         gbs = 8 * n / ms / 1e6
-        xd, yd = xs.double(), ys.double()
-        err = abs(o.item() - torch.dot(xd, yd).item()) / torch.dot(xd.abs(), yd.abs()).item()
+        xh, yh = xs.cpu().numpy(), ys.cpu().numpy()   # the oracle's FP64 dot
+        err = abs(o.item() - _orc().dot_f64(xh, yh)) / _orc().dot_f64(abs(xh), abs(yh))
         out[f"sdot_n{n}"] = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
                              "max_rel_err_vs_fp64": err}
-        del xs, ys, xd, yd
+        del xs, ys, xh, yh
     torch.cuda.empty_cache()
     # dgemm: native FP64 MFMA vs split-precision on FP8 MFMA, errors against
     # the compensated (Dot2) product, relative to (|A||B|)_ij, on both inputs
@@ -287,12 +410,11 @@ def bench_routines(torch, R, timer, dev, quick):
         R.hgemm(A, B, C)
         ms = timer.steady_ms(lambda: R.hgemm(A, B, C), max(3, it // 4))
         tf = 2 * n ** 3 / ms / 1e9
-        rows = torch.arange(0, n, n // 128, device=dev)
-        Ar, Bd = A[rows].double(), B.double()
-        err = ((C[rows].double() - Ar @ Bd).abs() / (Ar.abs() @ Bd.abs())).max().item()
+        rows, cols = sample_entries(n, n, 2048)
+        err = dot2_err(C.cpu().numpy()[rows, cols], A.cpu().numpy(), B.cpu().numpy(), rows, cols)
         out[f"hgemm_{n}"] = {"ms": round(ms, 4), "TFLOP/s": round(tf, 1), "frac_fp16_mfma": round(tf / FP16_PEAK, 4),
                              "config": repr(R.hgemm_default_config(n, n, n)), "max_rel_err_vs_fp64": err}
-        del A, B, C, Ar, Bd
+        del A, B, C
     # batched hgemm: 256 uneven problems, one persistent launch and its hipGraph
     from libthom_amd import tuner
     dims = R.batched_problem_dims()
@@ -380,17 +502,16 @@ def bench_routines(torch, R, timer, dev, quick):
         del ws
         byts = 2 * M * Kw + Kw // 8 * Nw * 4 + Kw // G * Nw * 2 + Kw // G * Nw // 2 + 2 * M * Nw
         flops = 2 * M * Nw * Kw
-        rows = torch.arange(0, M, max(1, M // 64), device=dev)
-        Wd = R.w4_dequant(Wt, St, Zt, G).double()   # bit-exact dequantisation
-        Ar = At[rows].double()
-        err = ((Ct[rows].double() - Ar @ Wd).abs() / (Ar.abs() @ Wd.abs())).max().item()
+        Wd = R.w4_dequant(Wt, St, Zt, G).cpu().numpy()   # bit-exact dequantisation
+        rows, cols = sample_entries(M, Nw, 2048)
+        err = dot2_err(Ct.cpu().numpy()[rows, cols], Aw, Wd, rows, cols)
         out[f"w4a16_M{M}"] = {"us": round(ms * 1e3, 2), "us_one_call_cold": round(ms_call * 1e3, 2),
                               "GB/s": round(byts / ms / 1e6, 1),
                               "frac_hbm": round(byts / ms / 1e6 / HBM_PEAK_GBS, 4),
                               "TFLOP/s": round(flops / ms / 1e9, 2),
                               "frac_fp16_mfma": round(flops / ms / 1e9 / FP16_PEAK, 4), "max_rel_err_vs_fp64": err,
                               "kernel": w4a16_kind_name(M, Nw, Kw)}
-        del Wd, Ar
+        del Wd
     if tuned:
         out["w4a16_tuned"] = tuned
     # grouped w4a16: many problems, each with its own weights, in one
@@ -478,10 +599,9 @@ def bench_w4a16_grouped(torch, R, inputs, np, timer, dev, it):
         torch.cuda.synchronize()
         err = 0.0
         for (a, w, s, z, C, G) in sets[0]:
-            rows = torch.arange(0, a.shape[0], max(1, a.shape[0] // 16), device=dev)
-            Wd = R.w4_dequant(w, s, z, G).double()
-            Ar = a[rows].double()
-            err = max(err, ((C[rows].double() - Ar @ Wd).abs() / (Ar.abs() @ Wd.abs())).max().item())
+            rows, cols = sample_entries(a.shape[0], C.shape[1], 512)
+            err = max(err, dot2_err(C.cpu().numpy()[rows, cols], a.cpu().numpy(),
+                                    R.w4_dequant(w, s, z, G).cpu().numpy(), rows, cols))
         flops = sum(2 * M * N * K for M, N, K, _ in dims)
         out[name] = {"us": round(us_g, 1), "TFLOP/s": round(flops / us_g / 1e6, 1),
                      "frac_fp16_mfma": round(flops / us_g / 1e6 / FP16_PEAK, 4),
@@ -518,7 +638,7 @@ def add_measured_fractions(routines, peaks):
         if "frac_fp16_mfma" in d:
             d["frac_measured_fp16"] = round(d["TFLOP/s"] / peaks["fp16_tflops"], 4)
         if "frac_fp64_mfma" in d:
-            d["frac_measured_fp64"] = round(d["TFLOP/s"] / peaks["fp64_ceiling_tflops"], 4)
+            d["frac_measured_fp64"] = round(d["TFLOP/s"] / peaks["fp64_tflops"], 4)
         if "frac_fp8_mfma" in d:
             d["frac_measured_fp8"] = round(d["fp8_TFLOP/s"] / peaks["fp8_tflops"], 4)
         for k, v in d.items():
@@ -594,6 +714,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="fewer iterations for the per-routine table")
     ap.add_argument("--no-routines", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweeps", action="store_true", help="skip the reference's SIZES sweeps (libthom bench)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run 5 plain 256 MiB copies (known traffic, to calibrate HBM counters)")
@@ -632,9 +753,8 @@ def main():
     ms = start.elapsed_time(end) / args.steps
     gbs = SG_BYTES / ms / 1e6
     log(f"sgemv 16384x8192: {ms * 1e3:.1f} us/step, {gbs:.0f} GB/s")
-    Ad, xd = A.double(), x.double()
-    sg_err = ((y.double() - Ad @ xd).abs() / (Ad.abs() @ xd.abs())).max().item()
-    del Ad, xd
+    rows, _ = sample_entries(M_SG, 1, 4096)
+    sg_err = dot2_err(y.cpu().numpy()[rows], A.cpu().numpy(), x.cpu().numpy()[:, None], rows, 0 * rows)
     torch.cuda.empty_cache()
 
     if args.copy_probe:
@@ -658,15 +778,14 @@ def main():
         log("per-routine table")
         routines = bench_routines(torch, R, timer, dev, args.quick)
         routines["vendor_reference"] = vendor_reference(torch, timer, dev)
-        # the FP64 MFMA loop probe runs below what a GEMM reaches: the FP64
-        # ceiling is the best rate any FP64 GEMM or the probe showed on this device
-        dg = routines.get("dgemm_4096", {}).get("a_normal", {}).get("native", {}).get("TFLOP/s", 0.0)
-        peaks["fp64_ceiling_tflops"] = max(peaks["fp64_tflops"], dg,
-                                           routines["vendor_reference"]["fp64_matmul_4096"]["TFLOP/s"])
         add_measured_fractions(routines, peaks)
+    sweeps = None
+    if not args.no_sweeps:
+        log("the reference's SIZES sweeps (libthom bench <routine> hip)")
+        sweeps = cli_sizes_sweeps()
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline (oracle sgemv, 1 thread)")
+        log("cpu baseline (oracle sgemv, all cores and 1 thread)")
         cpu = cpu_baseline(args.cpu_seconds)
 
     line = {
@@ -685,6 +804,7 @@ def main():
         "cpu_baseline": cpu,
         "measured_peaks": peaks,
         "routines": routines,
+        "sizes_sweeps": sweeps,
     }
     if peaks:
         line["roofline"]["measured_peak"] = peaks["hbm_read_gbs"]

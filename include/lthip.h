@@ -65,10 +65,11 @@ int lthip_copy_f32(float *dst, const float *src, size_t n, void *stream);
 /* ---- peak probes (no counterpart in the reference, which divides by the
  *      theoretical bandwidth of common.cpp:28-34): the rates this device
  *      delivers, for "fraction of measured peak".
- *      lthip_probe_mfma: `blocks` blocks of 4 waves, each wave `iters` rounds
- *      of independent MFMAs on registers; kind 0 = v_mfma_f32_32x32x16_f16,
- *      1 = v_mfma_scale_f32_32x32x64_f8f6f4 (fp8), 2 = v_mfma_f64_16x16x4_f64.
- *      sink holds blocks * 256 floats.  lthip_probe_mfma_flops gives the flops
+ *      lthip_probe_mfma: `blocks` blocks of 4 waves (kind 2: 8 waves), each
+ *      wave `iters` rounds of independent MFMAs on registers; kind 0 =
+ *      v_mfma_f32_32x32x16_f16, 1 = v_mfma_scale_f32_32x32x64_f8f6f4 (fp8),
+ *      2 = v_mfma_f64_16x16x4_f64 (4 x 4 operand registers, 16 accumulators).
+ *      sink holds blocks * 256 floats (kind 2: blocks * 512).  lthip_probe_mfma_flops gives the flops
  *      of one such launch.  lthip_probe_read_f32: read-only stream over x
  *      (16-byte aligned, n % 4 == 0); sink holds blocks * 256 floats. ------- */
 double lthip_probe_mfma_flops(int kind, int blocks, int iters);

@@ -1,4 +1,5 @@
 // Shared helpers of the CLI harness (the reference's routines/common.cpp).
+#include <chrono>
 #include <fstream>
 #include <thread>
 
@@ -81,8 +82,30 @@ void* workspace(const std::string& name, size_t bytes) {
 
 size_t workspace_size(const std::string& name) { return ws_map()[name].bytes; }
 
+static int g_cpu_threads = 0;   // 0: all host cores
+
+int cpu_threads() { return g_cpu_threads > 0 ? g_cpu_threads : host_cores(); }
+
+float cpu_time_ms(int threads, const std::function<void()>& fn) {
+    const int saved = g_cpu_threads;
+    g_cpu_threads = threads;
+    const auto t0 = std::chrono::steady_clock::now();
+    fn();
+    const auto t1 = std::chrono::steady_clock::now();
+    g_cpu_threads = saved;
+    return std::chrono::duration<float, std::milli>(t1 - t0).count();
+}
+
+void print_cpu_impl(const std::string& impl) {
+    if (impl == "cpu")
+        printf("# host implementation: 1 thread (host cores: %d, CPU: %s)\n", host_cores(), host_cpu_model().c_str());
+    else if (impl == "cpu_mt")
+        printf("# host implementation: %d threads (host cores: %d, CPU: %s)\n", host_cores(), host_cores(),
+               host_cpu_model().c_str());
+}
+
 void parallel_for(size_t n, const std::function<void(size_t, size_t)>& body) {
-    const size_t t = std::min<size_t>((size_t)host_cores(), n ? n : 1);
+    const size_t t = std::min<size_t>((size_t)cpu_threads(), n ? n : 1);
     if (t <= 1) {
         body(0, n);
         return;

@@ -41,7 +41,43 @@ static float dgemm_ozaki(const double* A, const double* B, double* C, size_t M, 
 
 static int slices_of(const std::string& name) { return name.rfind("ozaki", 0) == 0 ? atoi(name.c_str() + 5) : 0; }
 
+// Host implementations: the reference's CPU product (dgemm.cpp:122-133,
+// C = alpha A B + beta C in FP64) with the k loop outside the j loop (B read
+// along rows), on one thread ("cpu") or all host cores ("cpu_mt"), timed with
+// std::chrono; copies to and from the device are outside the timed region.
+static float dgemm_host(const double* A, const double* B, double* C, size_t M, size_t N, size_t K, double alpha,
+                        double beta, int threads) {
+    const std::vector<double> hA = cli::to_host(A, M * K), hB = cli::to_host(B, K * N);
+    std::vector<double> hC = cli::to_host(C, M * N);
+    const float ms = cli::cpu_time_ms(threads, [&] {
+        cli::parallel_for(M, [&](size_t lo, size_t hi) {
+            std::vector<double> acc(N);
+            for (size_t i = lo; i < hi; i++) {
+                std::fill(acc.begin(), acc.end(), 0.0);
+                for (size_t k = 0; k < K; k++) {
+                    const double a = hA[i * K + k];
+                    const double* b = &hB[k * N];
+                    for (size_t j = 0; j < N; j++) acc[j] += a * b[j];
+                }
+                for (size_t j = 0; j < N; j++) hC[i * N + j] = alpha * acc[j] + beta * hC[i * N + j];
+            }
+        });
+    });
+    cli::to_device(C, hC);
+    return ms;
+}
+static float dgemm_cpu(const double* A, const double* B, double* C, size_t M, size_t N, size_t K, double alpha,
+                       double beta) {
+    return dgemm_host(A, B, C, M, N, K, alpha, beta, 1);
+}
+static float dgemm_cpu_mt(const double* A, const double* B, double* C, size_t M, size_t N, size_t K, double alpha,
+                          double beta) {
+    return dgemm_host(A, B, C, M, N, K, alpha, beta, cli::host_cores());
+}
+
 static const std::unordered_map<std::string, DgemmImpl> dgemmImpls = {
+    {"cpu", dgemm_cpu},
+    {"cpu_mt", dgemm_cpu_mt},
 #if LIB_HAS(DGEMM_HIP)
     {"hip", dgemm_hip},
 #endif

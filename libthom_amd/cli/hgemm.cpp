@@ -46,11 +46,53 @@ static float hgemm_tuned(const uint16_t* A, const uint16_t* B, uint16_t* C, size
     return run_cfg(&c, A, B, C, M, N, K, alpha, beta);
 }
 
+// Host implementations: the reference's CPU product (hgemm.cu:137-152, an FP32
+// sum of fp16 products, C = fp16(alpha sum + beta C)) on one thread ("cpu") or
+// all host cores ("cpu_mt"), timed with std::chrono; copies to and from the
+// device are outside the timed region.
+static float hgemm_host(const uint16_t* A, const uint16_t* B, uint16_t* C, size_t M, size_t N, size_t K,
+                        uint16_t alpha, uint16_t beta, int threads) {
+    const std::vector<uint16_t> hA = cli::to_host(A, M * K), hB = cli::to_host(B, K * N);
+    std::vector<uint16_t> hC = cli::to_host(C, M * N);
+    const float al = cli::f16_to_f32(alpha), be = cli::f16_to_f32(beta);
+    const float ms = cli::cpu_time_ms(threads, [&] {
+        std::vector<float> fb(K * N);
+        cli::parallel_for(K * N, [&](size_t lo, size_t hi) {
+            for (size_t i = lo; i < hi; i++) fb[i] = cli::f16_to_f32(hB[i]);
+        });
+        cli::parallel_for(M, [&](size_t lo, size_t hi) {
+            std::vector<float> acc(N);
+            for (size_t i = lo; i < hi; i++) {
+                std::fill(acc.begin(), acc.end(), 0.0f);
+                for (size_t k = 0; k < K; k++) {
+                    const float a = cli::f16_to_f32(hA[i * K + k]);
+                    const float* b = &fb[k * N];
+                    for (size_t j = 0; j < N; j++) acc[j] += a * b[j];
+                }
+                for (size_t j = 0; j < N; j++)
+                    hC[i * N + j] = cli::f32_to_f16(al * acc[j] + be * cli::f16_to_f32(hC[i * N + j]));
+            }
+        });
+    });
+    cli::to_device(C, hC);
+    return ms;
+}
+static float hgemm_cpu(const uint16_t* A, const uint16_t* B, uint16_t* C, size_t M, size_t N, size_t K, uint16_t alpha,
+                       uint16_t beta) {
+    return hgemm_host(A, B, C, M, N, K, alpha, beta, 1);
+}
+static float hgemm_cpu_mt(const uint16_t* A, const uint16_t* B, uint16_t* C, size_t M, size_t N, size_t K,
+                          uint16_t alpha, uint16_t beta) {
+    return hgemm_host(A, B, C, M, N, K, alpha, beta, cli::host_cores());
+}
+
 static const std::unordered_map<std::string, HgemmImpl> hgemmImpls = {
 #if LIB_HAS(HGEMM_HIP)
     {"hip", hgemm_hip},
     {"tuned", hgemm_tuned},
 #endif
+    {"cpu", hgemm_cpu},
+    {"cpu_mt", hgemm_cpu_mt},
 };
 
 static cli::DevBuf<uint16_t> gpuA, gpuB, gpuC;

@@ -23,10 +23,36 @@ static float reduction_hip(const float* in, float* out, int N) {
     return cli::time_ms([&] { LTHIP_CHECK(lthip_reduce_sum_f32(in, (size_t)N, out, ws, need, cli::stream())); });
 }
 
+// Host implementations: the reference's CPU check (reduction.cpp:61-67, an
+// FP64 sum) on one thread ("cpu") or all host cores ("cpu_mt"), timed with
+// std::chrono; the input is copied to the host and the result back outside
+// the timed region.
+static float reduction_host(const float* in, float* out, int N, int threads) {
+    const std::vector<float> x = cli::to_host(in, (size_t)N);
+    double total = 0.0;
+    const float ms = cli::cpu_time_ms(threads, [&] {
+        std::mutex mu;
+        cli::parallel_for((size_t)N, [&](size_t lo, size_t hi) {
+            double s = 0.0;
+            for (size_t i = lo; i < hi; i++) s += (double)x[i];
+            std::lock_guard<std::mutex> g(mu);
+            total += s;
+        });
+    });
+    cli::to_device(out, std::vector<float>{(float)total});
+    return ms;
+}
+static float reduction_cpu(const float* in, float* out, int N) { return reduction_host(in, out, N, 1); }
+static float reduction_cpu_mt(const float* in, float* out, int N) {
+    return reduction_host(in, out, N, cli::host_cores());
+}
+
 static const std::unordered_map<std::string, ReductionImpl> reductionImpls = {
 #if LIB_HAS(REDUCTION_HIP)
     {"hip", reduction_hip},
 #endif
+    {"cpu", reduction_cpu},
+    {"cpu_mt", reduction_cpu_mt},
 };
 
 static cli::DevBuf<float> gpuX, gpuOut;

@@ -100,8 +100,27 @@ struct DevBuf {
 void* workspace(const std::string& name, size_t bytes);
 size_t workspace_size(const std::string& name);
 
-// Parallel loop over [0, n) on all host cores.
+// Parallel loop over [0, n) on cpu_threads() host threads.
 void parallel_for(size_t n, const std::function<void(size_t, size_t)>& body);
+// Threads parallel_for uses: all host cores, or the count a host
+// implementation ("cpu": 1, "cpu_mt": all cores) set for its timed region.
+int cpu_threads();
+// Runs fn with parallel_for on `threads` threads and returns its wall time in
+// milliseconds (std::chrono::steady_clock): the host implementations' timer.
+float cpu_time_ms(int threads, const std::function<void()>& fn);
+// Copies between device buffers and host vectors (outside any timed region).
+template <typename T>
+std::vector<T> to_host(const T* dev, size_t n) {
+    std::vector<T> h(n);
+    if (n) HIP_CHECK(hipMemcpy(h.data(), dev, sizeof(T) * n, hipMemcpyDeviceToHost));
+    return h;
+}
+template <typename T>
+void to_device(T* dev, const std::vector<T>& h) {
+    if (!h.empty()) HIP_CHECK(hipMemcpy(dev, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+}
+// The line printed before a host implementation runs: threads, host cores, CPU model.
+void print_cpu_impl(const std::string& impl);
 
 }  // namespace cli
 
@@ -156,6 +175,7 @@ struct Handlers {
             const std::string c = argv[4 + sizeDims + 1];
             if (c == "0" || c == "1") checkResult = c == "1";
         }
+        cli::print_cpu_impl(argv[3]);
         const float gflops = runnerFunc(it->second, sizes, nb_iter, 0, checkResult, true, argc, argv);
         printf("# GFLOP/s: %f\n", gflops);
     }
@@ -177,6 +197,7 @@ struct Handlers {
         }
         std::map<std::vector<int>, float> results;
         printf("-- Running %s -- %s\n", argv[2], argv[3]);
+        cli::print_cpu_impl(argv[3]);
         for (const std::vector<int>& sizes : SIZES) {
             printf("\nStarting benchmark for size %d...\n", sizes[0]);
             const float gflops = runnerFunc(it->second, sizes, BENCH_RUNS, WARMUP_RUNS, false, true, argc, argv);

@@ -23,10 +23,35 @@ static float sdot_hip(int n, const float* x, const float* y, float* result) {
     return cli::time_ms([&] { LTHIP_CHECK(lthip_sdot_f32((size_t)n, x, y, result, ws, need, cli::stream())); });
 }
 
+// Host implementations: the reference's CPU check (sdot.cpp:96-102, an FP64
+// dot) on one thread ("cpu") or all host cores ("cpu_mt"), timed with
+// std::chrono; copies to and from the device are outside the timed region.
+static float sdot_host(int n, const float* x, const float* y, float* result, int threads) {
+    const std::vector<float> hx = cli::to_host(x, (size_t)n), hy = cli::to_host(y, (size_t)n);
+    double total = 0.0;
+    const float ms = cli::cpu_time_ms(threads, [&] {
+        std::mutex mu;
+        cli::parallel_for((size_t)n, [&](size_t lo, size_t hi) {
+            double s = 0.0;
+            for (size_t i = lo; i < hi; i++) s += (double)hx[i] * (double)hy[i];
+            std::lock_guard<std::mutex> g(mu);
+            total += s;
+        });
+    });
+    cli::to_device(result, std::vector<float>{(float)total});
+    return ms;
+}
+static float sdot_cpu(int n, const float* x, const float* y, float* r) { return sdot_host(n, x, y, r, 1); }
+static float sdot_cpu_mt(int n, const float* x, const float* y, float* r) {
+    return sdot_host(n, x, y, r, cli::host_cores());
+}
+
 static const std::unordered_map<std::string, SdotImpl> sdotImpls = {
 #if LIB_HAS(SDOT_HIP)
     {"hip", sdot_hip},
 #endif
+    {"cpu", sdot_cpu},
+    {"cpu_mt", sdot_cpu_mt},
 };
 
 static cli::DevBuf<float> gpuX, gpuY, gpuOut;

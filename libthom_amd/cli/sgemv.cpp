@@ -28,10 +28,49 @@ static float sgemv_hip(int m, int n, const float* alpha, const float* A, int lda
     });
 }
 
+// Host implementations: the reference's CPU check (sgemv.cpp:132-147, an FP64
+// dot per row, then (float)(alpha dot + beta y)) on one thread ("cpu") or all
+// host cores ("cpu_mt"), timed with std::chrono; copies to and from the device
+// are outside the timed region.
+static float sgemv_host(int m, int n, const float* alpha, const float* A, int lda, const float* x, int incx,
+                        const float* beta, float* y, int incy, bool rowMajor, int threads) {
+    if (m <= 0) return 0.0f;
+    const size_t a_elems = n <= 0 ? 0 : (rowMajor ? (size_t)(m - 1) * lda + n : (size_t)(n - 1) * lda + m);
+    const std::vector<float> hA = cli::to_host(A, a_elems);
+    const std::vector<float> hx = cli::to_host(x, n <= 0 ? 0 : (size_t)(n - 1) * incx + 1);
+    std::vector<float> hy = cli::to_host(y, (size_t)(m - 1) * incy + 1);
+    const double al = cli::to_host(alpha, 1)[0], be = cli::to_host(beta, 1)[0];
+    const float ms = cli::cpu_time_ms(threads, [&] {
+        cli::parallel_for((size_t)m, [&](size_t lo, size_t hi) {
+            for (size_t row = lo; row < hi; row++) {
+                double dot = 0.0;
+                for (int col = 0; col < n; col++) {
+                    const double a = rowMajor ? hA[row * lda + col] : hA[(size_t)col * lda + row];
+                    dot += a * (double)hx[(size_t)col * incx];
+                }
+                float& yr = hy[row * incy];
+                yr = (float)(al * dot + be * (double)yr);
+            }
+        });
+    });
+    cli::to_device(y, hy);
+    return ms;
+}
+static float sgemv_cpu(int m, int n, const float* alpha, const float* A, int lda, const float* x, int incx,
+                       const float* beta, float* y, int incy, bool rowMajor) {
+    return sgemv_host(m, n, alpha, A, lda, x, incx, beta, y, incy, rowMajor, 1);
+}
+static float sgemv_cpu_mt(int m, int n, const float* alpha, const float* A, int lda, const float* x, int incx,
+                          const float* beta, float* y, int incy, bool rowMajor) {
+    return sgemv_host(m, n, alpha, A, lda, x, incx, beta, y, incy, rowMajor, cli::host_cores());
+}
+
 static const std::unordered_map<std::string, SgemvImpl> sgemvImpls = {
 #if LIB_HAS(SGEMV_HIP)
     {"hip", sgemv_hip},
 #endif
+    {"cpu", sgemv_cpu},
+    {"cpu_mt", sgemv_cpu_mt},
 };
 
 static cli::DevBuf<float> gpuA, gpuX, gpuY, gpuAlpha, gpuBeta;

@@ -82,11 +82,68 @@ static float w4a16_tuned(const Problem& p) {
     return run_cfg(p, &c);
 }
 
+// Host implementations: the reference's CPU path (quantize.cpp:58-65, W =
+// fp16((q - z) s) rounded once, then w4a16.cpp:178-184's product with an FP32
+// sum, C = fp16(sum)) on one thread ("cpu") or all host cores ("cpu_mt"),
+// timed with std::chrono; copies to and from the device are outside the timed
+// region.
+static float w4a16_host(const Problem& p, int threads) {
+    const size_t M = p.M, N = p.N, K = p.K, G = K / GROUP_SIZE;
+    const std::vector<uint16_t> hA = cli::to_host(p.A, M * K), hS = cli::to_host(p.scales, G * N);
+    const std::vector<uint32_t> hW = cli::to_host(p.W, (K / 8) * N);
+    const std::vector<uint32_t> hZ = p.qzeros ? cli::to_host(p.qzeros, G * (N / 8)) : std::vector<uint32_t>();
+    std::vector<uint16_t> hC(M * N);
+    const float ms = cli::cpu_time_ms(threads, [&] {
+        std::vector<float> w(K * N);
+        cli::parallel_for(K, [&](size_t lo, size_t hi) {
+            for (size_t k = lo; k < hi; k++) {
+                const size_t g = k / GROUP_SIZE;
+                for (size_t n = 0; n < N; n++) {
+                    const int q = (int)((hW[(k / 8) * N + n] >> (4 * (k % 8))) & 0xFu);
+                    const int z = p.qzeros ? (int)((hZ[g * (N / 8) + n / 8] >> (4 * (n % 8))) & 0xFu) : 8;
+                    w[k * N + n] = cli::f16_to_f32(cli::f32_to_f16((float)(q - z) * cli::f16_to_f32(hS[g * N + n])));
+                }
+            }
+        });
+        cli::parallel_for(M, [&](size_t lo, size_t hi) {
+            std::vector<float> acc(N);
+            for (size_t i = lo; i < hi; i++) {
+                std::fill(acc.begin(), acc.end(), 0.0f);
+                for (size_t k = 0; k < K; k++) {
+                    const float a = cli::f16_to_f32(hA[i * K + k]);
+                    const float* wk = &w[k * N];
+                    for (size_t n = 0; n < N; n++) acc[n] += a * wk[n];
+                }
+                for (size_t n = 0; n < N; n++) hC[i * N + n] = cli::f32_to_f16(acc[n]);
+            }
+        });
+    });
+    cli::to_device(p.C, hC);
+    return ms;
+}
+static cli::DevBuf<uint16_t> gpuWdq;   // the dequantised weights of the baseline
+
+// The baseline the reference times beside every implementation (w4a16.cpp:306,
+// cuBLAS fp16 on the dequantised weights): here the library's own FP16 GEMM on
+// the weights dequantised once into gpuWdq (outside the timed region).
+static float dense_fp16_on_dequantised(const Problem& p) {
+    const uint16_t one = 0x3C00, zero = 0x0000;
+    const size_t need = lthip_hgemm_workspace_size(p.M, p.N, p.K, nullptr);
+    void* ws = need ? cli::workspace("w4a16_dense", need) : nullptr;
+    return cli::time_ms([&] {
+        LTHIP_CHECK(lthip_hgemm_f16(p.A, gpuWdq.p, p.C, p.M, p.N, p.K, one, zero, nullptr, ws, need, cli::stream()));
+    });
+}
+
+static float w4a16_cpu(const Problem& p) { return w4a16_host(p, 1); }
+static float w4a16_cpu_mt(const Problem& p) { return w4a16_host(p, cli::host_cores()); }
+
 static const std::unordered_map<std::string, W4A16Function> w4a16Impls = {
 #if LIB_HAS(W4A16_HIP)
     {"hip", w4a16_hip},       {"stream", w4a16_stream}, {"decode", w4a16_decode}, {"skinny", w4a16_skinny},
     {"tiled", w4a16_tiled},   {"pipelined", w4a16_pipelined}, {"tuned", w4a16_tuned},
 #endif
+    {"cpu", w4a16_cpu},       {"cpu_mt", w4a16_cpu_mt},
 };
 
 static cli::DevBuf<uint16_t> gpuA, gpuScales, gpuC;
@@ -209,7 +266,9 @@ static bool compareToReference(int M, int N) {
 }
 
 struct Metrics {
-    float time_us, gflops, gbps, pct_peak_bw, baseline_time_us, speedup_vs_baseline;
+    // cublas_* (the reference's column names): the dense FP16 GEMM on the
+    // dequantised weights, the reference's baseline, run by this library
+    float time_us, gflops, gbps, pct_peak_bw, cublas_time_us, speedup_vs_cublas;
 };
 static std::map<std::vector<int>, Metrics> metrics;
 
@@ -257,11 +316,13 @@ static float runner(W4A16Function impl, std::vector<int> sizes, int nb_runs, int
     const Problem p{gpuA.p, gpuW.p, gpuScales.p, symmetric ? nullptr : gpuZ.p, gpuC.p, (size_t)M, (size_t)N,
                     (size_t)K};
     const auto times = measure(impl, p, nb_runs, nb_warmup_runs, checkResult);
-    // baseline: the untuned size-based choice (the reference compares with cuBLAS fp16, w4a16.cpp:306)
-    const auto base = measure(w4a16_hip, p, nb_runs, nb_warmup_runs, false);
+    // baseline: a dense FP16 GEMM on the dequantised weights (w4a16.cpp:306)
+    gpuWdq.alloc((size_t)K * N);
+    LTHIP_CHECK(lthip_w4_dequant(p.W, p.scales, p.qzeros, gpuWdq.p, (size_t)K, (size_t)N, GROUP_SIZE, cli::stream()));
+    const auto base = measure(dense_fp16_on_dequantised, p, nb_runs, nb_warmup_runs, false);
     if (printEach)
         for (int i = 0; i < nb_runs; i++)
-            printf("[internal][iter %d]: %f GFLOP/s | %.2f us (untuned: %.2f us)\n", i,
+            printf("[internal][iter %d]: %f GFLOP/s | %.2f us (dense fp16 on dequantised W: %.2f us)\n", i,
                    2.0 * M * N * K / (times[i] * 1e6), times[i] * 1e3f, base[i] * 1e3f);
     const float ms = median(times), bms = median(base);
     Metrics m;
@@ -269,27 +330,28 @@ static float runner(W4A16Function impl, std::vector<int> sizes, int nb_runs, int
     m.gflops = (float)(2.0 * M * N * K / (ms * 1e6));
     m.gbps = (float)(bytesMoved(M, N, K) / (ms * 1e6));
     m.pct_peak_bw = 100.0f * m.gbps / (float)cli::kHbmPeakGBs;
-    m.baseline_time_us = bms * 1e3f;
-    m.speedup_vs_baseline = bms / ms;
+    m.cublas_time_us = bms * 1e3f;
+    m.speedup_vs_cublas = bms / ms;
     metrics[sizes] = m;
-    printf("# median: %.2f us | %.1f GB/s (%.1f%% of %.0f GB/s) | %.1f TFLOP/s (%.1f%% of FP16 MFMA) | untuned: %.2f us\n",
+    printf("# median: %.2f us | %.1f GB/s (%.1f%% of %.0f GB/s) | %.1f TFLOP/s (%.1f%% of FP16 MFMA) | dense fp16 on dequantised W: %.2f us | speedup: %.2fx\n",
            m.time_us, m.gbps, m.pct_peak_bw, cli::kHbmPeakGBs, m.gflops / 1000.0,
-           100.0 * m.gflops / 1000.0 / cli::kFp16MfmaTflops, m.baseline_time_us);
+           100.0 * m.gflops / 1000.0 / cli::kFp16MfmaTflops, m.cublas_time_us, m.speedup_vs_cublas);
     gpuA.free();
     gpuW.free();
     gpuScales.free();
     gpuZ.free();
     gpuC.free();
+    gpuWdq.free();
     return m.gflops;
 }
 
 static void exportCSV(const std::map<std::vector<int>, float>& results, const std::string& filename) {
     std::ofstream file(filename);
-    file << "M,N,K,time_us,gflops,gbps,pct_peak_bw,baseline_time_us,speedup_vs_baseline\n";
+    file << "M,N,K,time_us,gflops,gbps,pct_peak_bw,cublas_time_us,speedup_vs_cublas\n";   // w4a16.cpp:343
     for (const auto& [s, g] : results) {
         const Metrics& m = metrics.at(s);
         file << s[0] << "," << s[1] << "," << s[2] << "," << m.time_us << "," << m.gflops << "," << m.gbps << ","
-             << m.pct_peak_bw << "," << m.baseline_time_us << "," << m.speedup_vs_baseline << "\n";
+             << m.pct_peak_bw << "," << m.cublas_time_us << "," << m.speedup_vs_cublas << "\n";
     }
 }
 

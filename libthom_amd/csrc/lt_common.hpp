@@ -61,19 +61,32 @@ __device__ __forceinline__ void st_nt(float4* p, float4 a) {
 // Each XCD of an MI355X has its own L2, and the L2s are not coherent with one
 // another, so an agent-scope release/acquire fence compiles to a write-back /
 // invalidate of the whole L2 (buffer_wbl2 / buffer_inv sc1) -- per block, that
-// is microseconds.  Cross-block data instead uses relaxed agent-scope atomic
-// loads and stores (sc1: written through / read past the L2), an explicit
-// s_waitcnt so the stores have landed before the block takes its ticket, and a
-// relaxed atomic ticket counter.  The block holding the last ticket then reads
-// the published data with sc1 loads.
+// is microseconds.  Cross-block data instead moves only through agent-scope
+// atomics, which are coherent across XCDs on their own:
+//   publish  = relaxed agent-scope atomic store (global_store ... sc1: written
+//              through past the XCD's L2),
+//   drain    = s_waitcnt vmcnt(0) lgkmcnt(0) with a memory clobber: the
+//              hardware waits until this thread's stores are acknowledged
+//              (gfx9 counts stores in vmcnt), and the compiler may not move
+//              any memory access across it,
+//   ticket   = relaxed agent-scope atomic add, issued after the drain (and,
+//              where several threads publish, after a barrier behind all
+//              their drains),
+//   collect  = relaxed agent-scope atomic load (sc1), issued only by the block
+//              that received the last ticket, after the ticket value has come
+//              back and been passed through LDS and a barrier.
+// No data crosses blocks outside these atomics, so no release/acquire is
+// needed to order other memory traffic; the only orders required are
+// publish-before-ticket (drain) and ticket-before-collect (the control
+// dependency and the barrier).
 __device__ __forceinline__ void publish(float* p, float v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ float collect(const float* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// all of this thread's memory operations have completed
-__device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
+// all of this thread's memory operations have completed; a compiler barrier too
+__device__ __forceinline__ void drain() { asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory"); }
 __device__ __forceinline__ unsigned take_ticket(unsigned* counter) {
     return __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

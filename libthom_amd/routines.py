@@ -382,7 +382,7 @@ def measured_peaks(device=None, read_bytes: int = 2 << 30) -> dict:
     caches)."""
     device = torch.device(device or "cuda:0")
     cus = torch.cuda.get_device_properties(device).multi_processor_count
-    sink = torch.empty(cus * 16 * 256, dtype=torch.float32, device=device)
+    sink = torch.empty(cus * 16 * 512, dtype=torch.float32, device=device)   # 16 blocks of <= 512 threads per CU
     stream = _stream(device)
 
     def timed(fn, reps=3):

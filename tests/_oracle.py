@@ -13,7 +13,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
-THREADS = min(8, os.cpu_count() or 1)
+try:
+    _NCPU = len(os.sched_getaffinity(0))
+except AttributeError:
+    _NCPU = os.cpu_count() or 1
+THREADS = max(1, min(16, _NCPU))   # all usable CPUs, at most a GPU box's share of 16
 
 if not os.path.exists(SO):
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, capture_output=True)

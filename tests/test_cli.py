@@ -14,10 +14,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "libthom_amd", "bin", "libthom")
 
 
-def run(*args, timeout=300, env=None):
+def run(*args, timeout=300, env=None, cwd=None):
     if not os.path.exists(EXE):
         pytest.fail("libthom CLI not built (make -C libthom_amd)")
-    return subprocess.run([EXE, *map(str, args)], capture_output=True, text=True, timeout=timeout, env=env)
+    return subprocess.run([EXE, *map(str, args)], capture_output=True, text=True, timeout=timeout, env=env, cwd=cwd)
 
 
 def test_usage_and_unknown_routine():
@@ -48,6 +48,10 @@ def test_missing_arguments_list_implementations():
         assert r.returncode == 1 and ("Usage" in r.stderr or "need positive" in r.stderr), args
     r = run("test", "hgemm_batched", "bogus")
     assert r.returncode == 1 and "Usage" in r.stderr
+    # every routine registers the host implementations beside the HIP ones
+    for routine in ("reduction", "sdot", "sgemv", "dgemm", "hgemm", "w4a16"):
+        r = run("test", routine)
+        assert r.returncode == 1 and " cpu " in r.stderr + " " and "cpu_mt" in r.stderr, (routine, r.stderr)
 
 
 @pytest.mark.gpu
@@ -102,3 +106,43 @@ def test_cli_batched_and_tune(tmp_path):
     assert rows[0] == "M,N,K,bm,bn,bk,split_k,stages,ms" and len(rows) > 10
     r = run("test", "hgemm", "tuned", 256, 256, 512, 2, env=env)
     assert r.returncode == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("impl", ["cpu", "cpu_mt"])
+@pytest.mark.parametrize("args", [
+    ("reduction", 1_000_003, 2),
+    ("sdot", 1_000_000, 2),
+    ("sgemv", 1024, 512, 2, 1, "row"),
+    ("sgemv", 600, 300, 2, 1, "col"),
+    ("dgemm", 256, 192, 128, 2),
+    ("hgemm", 256, 136, 192, 2),
+    ("w4a16", 16, 512, 1024, 2),
+    ("w4a16", 3, 256, 512, 2, 1, "sym"),
+])
+def test_cli_host_implementations(args, impl):
+    # the reference's CPU path registered as an implementation and run through
+    # the routine's own runner: its result passes the same check
+    routine, *rest = args
+    r = run("test", routine, impl, *rest)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "# host implementation: " in r.stdout and "host cores" in r.stdout and "CPU:" in r.stdout
+    assert "# GFLOP/s" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cli_bench_mode_writes_reference_csv(tmp_path):
+    # bench mode runs the routine's SIZES and writes bench_<routine>_<impl>.csv
+    # (routines.h:157-158) with the reference's columns (reduction.cpp:132,
+    # w4a16.cpp:343)
+    cases = {("reduction", "hip"): ("size,gflops", 8),
+             ("w4a16", "hip"): ("M,N,K,time_us,gflops,gbps,pct_peak_bw,cublas_time_us,speedup_vs_cublas", 13)}
+    for (routine, impl), (header, rows) in cases.items():
+        print(f"libthom bench {routine} {impl}", flush=True)
+        r = run("bench", routine, impl, timeout=600, cwd=str(tmp_path))
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        lines = (tmp_path / f"bench_{routine}_{impl}.csv").read_text().splitlines()
+        assert lines[0] == header and len(lines) == rows + 1, lines[:3]
+        for ln in lines[1:]:
+            vals = [float(v) for v in ln.split(",")]
+            assert len(vals) == len(header.split(",")) and all(v > 0 for v in vals), ln

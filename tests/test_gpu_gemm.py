@@ -61,6 +61,14 @@ def test_dgemm_config_sampled_vs_dot2(dev, orc):
     As = A * (2.0 ** u)[:, None]
     Cs = R.dgemm(t(As, dev), t(B, dev)).cpu().numpy()
     assert np.array_equal(Cs, C * (2.0 ** u)[:, None])
+    # input set (b): rows of A and columns of B scaled, against the CPU Dot2 oracle
+    v = inputs.int_uniform(n, -20, 20, 1234, inputs.S_DGEMM_CS)
+    Bs = B * (2.0 ** v)[None, :]
+    Cb = R.dgemm(t(As, dev), t(Bs, dev)).cpu().numpy()
+    assert np.array_equal(Cb, C * (2.0 ** u)[:, None] * (2.0 ** v)[None, :])
+    hi, lo = orc.dgemm_dot2(As, Bs, rows, cols)
+    scale_b = np.einsum("ij,ji->i", np.abs(As[rows]), np.abs(Bs[:, cols]))
+    assert (np.abs(Cb[rows, cols] - (hi + lo)) / scale_b).max() < 1e-14
 
 
 def hgemm_check(got, ref, A, B):

@@ -93,3 +93,40 @@ def test_batched_configuration(dev):
         A, B, C, split = probs[i]
         assert torch.equal(C, single(A, B, cfg, split)), dims[i]
     plan.close()
+
+
+def _padding(M, N, bm, bn):
+    return 1.0 - (M * N) / (((M + bm - 1) // bm) * bm * ((N + bn - 1) // bn) * bn)
+
+
+def test_batched_configuration_vs_oracle(dev, orc):
+    """The configured batched launch (256 log-uniform problems, the tuned tile
+    and split policy) against the oracle (hgemm.cu:137-152) with the reference's
+    criterion (hgemm.cu:165-168), on sampled rows of at least 16 problems: the
+    largest K, every split-K problem, the six most edge-padded, the largest and
+    smallest by flops, and random ones."""
+    from libthom_amd import tuner
+    from test_gpu_gemm import hgemm_check
+    dims = R.batched_problem_dims()
+    cfg, policy = tuner.best_batched(len(dims))
+    probs, _ = tuner.batched_problems(dims, dev, policy, cfg)
+    plan = R.GroupedHgemm(probs, cfg)
+    plan.run()
+    torch.cuda.synchronize()
+    flops = [M * N * K for M, N, K in dims]
+    pick = {int(np.argmax([K for _, _, K in dims])), int(np.argmax(flops)), int(np.argmin(flops))}
+    pick |= {i for i, p in enumerate(probs) if p[3] > 1}
+    pick |= set(int(i) for i in np.argsort([-_padding(M, N, cfg.bm, cfg.bn) for M, N, _ in dims])[:6])
+    rng = np.random.default_rng(3)
+    while len(pick) < 20:
+        pick.add(int(rng.integers(0, len(dims))))
+    for i in sorted(pick):
+        A, B, C, split = probs[i]
+        M, N, K = dims[i]
+        rows = np.unique(np.concatenate([[0, M - 1], rng.integers(0, M, 14)]))
+        An = A.cpu().numpy()[rows]
+        Bn = B.cpu().numpy()
+        ref = orc.hgemm(An, Bn)
+        print(f"batched problem {i}: {M}x{N}x{K} split {split}", flush=True)
+        hgemm_check(C.cpu().numpy()[rows], ref, An, Bn)
+    plan.close()
