@@ -622,7 +622,9 @@ def w4a16_kind_name(M, N, K):
         return "decode (kind 4)"
     if M <= 64:
         return "skinny MFMA (kind 5)"
-    return "pipelined MFMA tile (kind 3)" if N % 32 == 0 else "tiled MFMA (kind 2)"
+    if N % 32 != 0:
+        return "tiled MFMA (kind 2)"
+    return "pipelined MFMA tile (kind 3, " + ("64x128)" if M <= 256 else "128x128 or 256x256)")
 
 
 def add_measured_fractions(routines, peaks):

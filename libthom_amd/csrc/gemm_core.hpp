@@ -79,6 +79,26 @@ __device__ __forceinline__ uint4 dequant8(uint32_t w, uint32_t zz, uint32_t ss) 
     return r;
 }
 
+// The same eight weights in natural k order [0..7] (MFMA fragment order, so
+// the A operand needs no permutation): pair p = (k 2p, k 2p + 1) comes from
+// byte p of w, copied into bytes 0 and 2 (one v_perm_b32), then
+// (y & 0x00F0000F) | 0x54006400 = fp16 (1024 + q_2p, 64 + q_2p+1): the high
+// nibble sits under the exponent of 64, whose ulp is 1/16.  zz = the pair
+// (0x6400 | z, 0x5400 | 16 z) = fp16 (1024 + z, 64 + z), so the subtraction
+// leaves (q - z) exactly and the multiplication by (s, s) rounds once.
+__device__ __forceinline__ uint4 dequant8_nat(uint32_t w, uint32_t zz, uint32_t ss) {
+    const __half2 z2 = u32_as_h2(zz), s2 = u32_as_h2(ss);
+    const uint32_t m = 0x00F0000Fu, c = 0x54006400u;
+    uint4 r;
+    r.x = h2_as_u32(__hmul2(__hsub2(u32_as_h2(and_or(__builtin_amdgcn_perm(w, w, 0x0c000c00u), m, c)), z2), s2));
+    r.y = h2_as_u32(__hmul2(__hsub2(u32_as_h2(and_or(__builtin_amdgcn_perm(w, w, 0x0c010c01u), m, c)), z2), s2));
+    r.z = h2_as_u32(__hmul2(__hsub2(u32_as_h2(and_or(__builtin_amdgcn_perm(w, w, 0x0c020c02u), m, c)), z2), s2));
+    r.w = h2_as_u32(__hmul2(__hsub2(u32_as_h2(and_or(__builtin_amdgcn_perm(w, w, 0x0c030c03u), m, c)), z2), s2));
+    return r;
+}
+// zz operand of dequant8_nat for zero-point z
+__device__ __forceinline__ uint32_t zz_nat(uint32_t z) { return (0x6400u | z) | ((0x5400u | (z << 4)) << 16); }
+
 // ---- A operand: row-major fp16 M x K (lda), 16-byte chunks ---------------
 template <int BM, int BK, bool PERM>
 struct ALoader {
@@ -227,42 +247,96 @@ struct TileOut {
     unsigned* counter;   // split-K: this tile's arrival counter
 };
 
-// Read-back by the last split of a tile: for the 16 accumulator rows rows[r]
-// of one MFMA tile column col, C = alpha * sum_sp partial[sp] (+ beta * C),
-// the splits summed in order.  Loads are issued two splits (32 values) at a
-// time from clamped addresses, so no load waits for the previous one.
-__device__ __forceinline__ void collect_store16(const TileOut& out, int M, int N, int splits, const int* rows,
-                                                int col) {
-    const int cc = col < N ? col : N - 1;
-    long off[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) off[r] = (long)(rows[r] < M ? rows[r] : M - 1) * N + cc;
-    float sum[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) sum[r] = 0.0f;
+// The last block of a split tile writes C[m0:m0+BM, n0:n0+BN] = alpha * (sum
+// of the splits' partials, in split order, starting from 0) (+ beta * C),
+// element by element over the tile.  Nothing of the MFMA accumulator layout
+// stays live here, so the epilogue adds no registers to the tile's main loop
+// (a read-back per accumulator fragment made the 256 x 256 tiles spill 100-200
+// VGPRs).  Partials are read with the cache policy of an agent-scope atomic
+// load (sc1, the encoding the compiler gives collect()), 16 bytes at a time:
+// each thread takes four column quads of the tile and issues the loads of four
+// splits of all of them (16 x 16 bytes) before adding, so the read-back costs
+// a few memory round trips, not one per element.  N % 4 != 0 (or partials past
+// 2 GiB) falls back to one float per load.
+template <int BM, int BN, int THREADS>
+__device__ __forceinline__ void collect_tile(const TileOut& out, int M, int N, int splits, int m0, int n0) {
+    static_assert(BN % 4 == 0, "column quads");
+    constexpr int Q = BN / 4, QUADS = BM * Q, QU = 4, SB = 4, SC1 = 16;
     const long slice = (long)M * N;
-    for (int sp = 0; sp < splits; sp += 2) {
-        const int sp1 = sp + 1 < splits ? sp + 1 : sp;
-        float v0[16], v1[16];
+    auto store4 = [&](int row, int col, const float* v, int cols) {
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            v0[r] = collect(out.partial + sp * slice + off[r]);
-            v1[r] = collect(out.partial + sp1 * slice + off[r]);
+        for (int c = 0; c < 4; c++) {
+            if (c >= cols) break;
+            __half* cp = out.C + (long)row * out.ldc + col + c;
+            float x = out.alpha * v[c];
+            if (out.beta != 0.0f) x = fmaf(out.beta, __half2float(*cp), x);
+            *cp = __float2half_rn(x);
         }
+    };
+    if (N % 4 == 0 && (long)splits * slice * 4 <= 0x7FFFFFFFL) {
+        const __amdgpu_buffer_rsrc_t pr =
+            __builtin_amdgcn_make_buffer_rsrc((void*)out.partial, 0, (int)((long)splits * slice * 4), 0x00020000);
+        for (int e0 = threadIdx.x; e0 < QUADS; e0 += THREADS * QU) {
+            int off[QU], rows[QU], cols[QU];
+            bool ok[QU];
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            sum[r] += v0[r];
-            if (sp + 1 < splits) sum[r] += v1[r];
+            for (int u = 0; u < QU; u++) {
+                const int e = e0 + u * THREADS;
+                rows[u] = m0 + e / Q;
+                cols[u] = n0 + 4 * (e % Q);
+                ok[u] = e < QUADS && rows[u] < M && cols[u] < N;
+                off[u] = ok[u] ? (rows[u] * N + cols[u]) * 4 : 0;
+            }
+            v4f sum[QU];
+#pragma unroll
+            for (int u = 0; u < QU; u++) sum[u] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int sp = 0; sp < splits; sp += SB) {
+                v4f v[SB][QU];
+#pragma unroll
+                for (int b = 0; b < SB; b++) {
+                    const int q = sp + b < splits ? sp + b : splits - 1;
+#pragma unroll
+                    for (int u = 0; u < QU; u++)
+                        v[b][u] = __builtin_amdgcn_raw_buffer_load_b128(pr, off[u], (int)(q * slice * 4), SC1);
+                }
+#pragma unroll
+                for (int b = 0; b < SB; b++)
+                    if (sp + b < splits) {
+#pragma unroll
+                        for (int u = 0; u < QU; u++) sum[u] += v[b][u];
+                    }
+            }
+#pragma unroll
+            for (int u = 0; u < QU; u++)
+                if (ok[u]) {
+                    const float w[4] = {sum[u][0], sum[u][1], sum[u][2], sum[u][3]};
+                    store4(rows[u], cols[u], w, 4);
+                }
         }
+        return;
     }
-    if (col >= N) return;
+    for (int e = threadIdx.x; e < QUADS; e += THREADS) {
+        const int row = m0 + e / Q, col0 = n0 + 4 * (e % Q);
+        if (row >= M || col0 >= N) continue;
+        const long off = (long)row * N + col0;
+        const int ncols = N - col0 < 4 ? N - col0 : 4;
+        float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int sp = 0; sp < splits; sp += SB) {
+            float v[SB][4];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        if (rows[r] >= M) continue;
-        __half* cp = out.C + (long)rows[r] * out.ldc + col;
-        float v = out.alpha * sum[r];
-        if (out.beta != 0.0f) v = fmaf(out.beta, __half2float(*cp), v);
-        *cp = __float2half_rn(v);
+            for (int b = 0; b < SB; b++) {
+                const int q = sp + b < splits ? sp + b : splits - 1;
+#pragma unroll
+                for (int c = 0; c < 4; c++) v[b][c] = collect(out.partial + q * slice + off + (c < ncols ? c : 0));
+            }
+#pragma unroll
+            for (int b = 0; b < SB; b++)
+                if (sp + b < splits) {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) sum[c] += v[b][c];
+                }
+        }
+        store4(row, col0, sum, ncols);
     }
 }
 
@@ -370,15 +444,7 @@ __device__ __forceinline__ void gemm_tile(__half* smem, const __half* __restrict
     if (threadIdx.x == 0) last_flag = (take_ticket(out.counter) == (unsigned)splits - 1);
     __syncthreads();
     if (!last_flag) return;
-#pragma unroll
-    for (int i = 0; i < TM; i++)
-#pragma unroll
-        for (int j = 0; j < TN; j++) {
-            int rows[16];
-#pragma unroll
-            for (int r = 0; r < 16; r++) rows[r] = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
-            collect_store16(out, M, N, splits, rows, n0 + wn + 32 * j + fcol);
-        }
+    collect_tile<BM, BN, 256>(out, M, N, splits, m0, n0);
     if (threadIdx.x == 0) reset_counter(out.counter);
 }
 

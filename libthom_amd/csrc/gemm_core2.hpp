@@ -387,15 +387,7 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
     if (threadIdx.x == 0) *last_split = (take_ticket(out.counter) == (unsigned)splits - 1);
     __syncthreads();
     if (!*last_split) return;
-#pragma unroll
-    for (int i = 0; i < TM; i++)
-#pragma unroll
-        for (int j = 0; j < TN; j++) {
-            int rows[16];
-#pragma unroll
-            for (int r = 0; r < 16; r++) rows[r] = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
-            collect_store16(out, M, N, splits, rows, n0 + wn + 32 * j + fcol);
-        }
+    collect_tile<C::BM, C::BN, C::THREADS>(out, M, N, splits, m0, n0);
     if (threadIdx.x == 0) reset_counter(out.counter);
 }
 

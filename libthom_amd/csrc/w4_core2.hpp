@@ -14,8 +14,8 @@
 //
 // One uint32 of W is one MFMA B fragment: lane l of n-tile j reads row
 // 2 kk + l/32, column wn + 32 j + l%32, and dequantises it in registers to
-// fp16((q - z) * s) in k order [0,4,1,5,2,6,3,7] (dequant8, bit-exact); the A
-// fragment is permuted the same way, which leaves every dot product unchanged.
+// fp16((q - z) * s) in natural k order (dequant8_nat, one rounding), so the A
+// fragment goes from LDS to the MFMA as it is.
 //
 // Requirements (checked by the caller): group % 64 == 0, K % group == 0,
 // N % 32 == 0, M >= 1; A, W, scales and qzeros 16-byte aligned.
@@ -143,8 +143,7 @@ struct W4Frags {
             const int row = wm + 32 * i + frow;
             const int kc = (2 * kk + fkc) ^ swzA2<C::BK>(row);
             const uint4 raw = *reinterpret_cast<const uint4*>(sa + row * (C::BK * 2) + kc * 16);
-            const uint4 pr = perm_k8(raw);
-            __builtin_memcpy(&a[i], &pr, 16);
+            __builtin_memcpy(&a[i], &raw, 16);
         }
         const int r = 2 * kk + fkc;
 #pragma unroll
@@ -155,11 +154,11 @@ struct W4Frags {
     }
 };
 
-// B fragment j of f: fp16((q - z) * s) in the perm_k8 k order; zz = fp16(1024
-// + z) and s2 = the scale, both in both halves
+// B fragment j of f: fp16((q - z) * s) in natural k order; zz = zz_nat(z),
+// s2 = the scale in both halves
 template <class C>
 __device__ __forceinline__ v8h dq(const W4Frags<C>& f, int j, const uint32_t* zz, const uint32_t* s2) {
-    const uint4 d = dequant8(f.w[j], zz[j], s2[j]);
+    const uint4 d = dequant8_nat(f.w[j], zz[j], s2[j]);
     v8h b;
     __builtin_memcpy(&b, &d, 16);
     return b;
@@ -176,7 +175,7 @@ __device__ __forceinline__ void w4_read_sz(const char* ss, int cg, bool sym, uin
         const uint32_t s = *reinterpret_cast<const uint16_t*>(blk + 2 * c);
         uint32_t z = 8;
         if (!sym) z = (*reinterpret_cast<const uint32_t*>(blk + C::SZ_SCALE + 4 * (c >> 3)) >> (4 * (c & 7))) & 0xFu;
-        zz[j] = (0x6400u + z) * 0x10001u;
+        zz[j] = zz_nat(z);
         s2[j] = s | (s << 16);
     }
 }
@@ -303,15 +302,7 @@ __device__ __forceinline__ void w4_tile2(char* smem, const __half* __restrict__ 
     if (threadIdx.x == 0) *last_split = (take_ticket(out.counter) == (unsigned)splits - 1);
     __syncthreads();
     if (!*last_split) return;
-#pragma unroll
-    for (int i = 0; i < TM; i++)
-#pragma unroll
-        for (int j = 0; j < TN; j++) {
-            int rows[16];
-#pragma unroll
-            for (int r = 0; r < 16; r++) rows[r] = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
-            collect_store16(out, M, N, splits, rows, n0 + wn + 32 * j + fcol);
-        }
+    collect_tile<C::BM, C::BN, C::THREADS>(out, M, N, splits, m0, n0);
     if (threadIdx.x == 0) reset_counter(out.counter);
 }
 

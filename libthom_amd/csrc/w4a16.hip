@@ -746,7 +746,8 @@ struct Tiled2Entry {
     {bm, bn, Core2W4<bm, bn, st, wm, wn>::SMEM, w4a16_tiled2_kernel<Core2W4<bm, bn, st, wm, wn>>, \
      Core2W4<bm, bn, st, wm, wn>::THREADS},
 const Tiled2Entry kTiled2[] = {LT_W4T2(256, 256, 3, 2, 4) LT_W4T2(256, 128, 3, 2, 2) LT_W4T2(128, 256, 3, 2, 4)
-                                   LT_W4T2(128, 128, 3, 2, 2) LT_W4T2(64, 256, 3, 1, 8)};
+                                   LT_W4T2(128, 128, 3, 2, 2) LT_W4T2(64, 256, 3, 1, 8) LT_W4T2(64, 128, 3, 1, 4)
+                                       LT_W4T2(64, 64, 3, 1, 2)};
 #undef LT_W4T2
 constexpr int kNumTiled2 = sizeof(kTiled2) / sizeof(kTiled2[0]);
 
@@ -793,15 +794,17 @@ void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_
             c.tile_m = 0;
             c.split_k = 1;
         } else if (N % 32 == 0) {
-            // pipelined tiles: 128 x 128, or 256 x 256 (8 waves) once those
-            // cover half the CUs (measured at N = 11008: 256 x 256 faster
-            // from M = 1024; the 128 x 256 tiles were slower than either at
-            // M = 256-512); the launch shares the last partial round of
-            // tiles out by splitting K
+            // pipelined tiles: 64 x 128 up to M = 256 (at N = 11008, K = 4096:
+            // M = 128 in 33 against 43 us for 128 x 128 with a K-split tail,
+            // M = 256 in 49 against 56 us), then 128 x 128, or 256 x 256
+            // (8 waves) once those cover half the CUs (256 x 256 faster from
+            // M = 1024; the 128 x 256 tiles were slower than either at M =
+            // 256-512); the launch shares the last partial round of tiles
+            // out by splitting K
             c.kind = 3;
             const long tiles256 = (long)((M + 255) / 256) * (long)((N + 255) / 256);
             const bool big = M > 512 && 2 * tiles256 >= cu_count();
-            c.tile_m = big ? 256 : 128;
+            c.tile_m = big ? 256 : (M <= 256 ? 64 : 128);
             c.tile_n = big ? 256 : 128;
             c.split_k = 1;
         } else {

@@ -126,7 +126,8 @@ def test_w4a16_symmetric_zero_point(dev, orc):
 
 KIND3 = [W4A16Config(3, 256, 256, 1), W4A16Config(3, 128, 256, 1), W4A16Config(3, 256, 128, 1),
          W4A16Config(3, 128, 128, 1), W4A16Config(3, 128, 128, 3), W4A16Config(3, 256, 256, 2),
-         W4A16Config(3, 256, 64, 1), W4A16Config(3, 256, 64, 2)]
+         W4A16Config(3, 256, 64, 1), W4A16Config(3, 256, 64, 2), W4A16Config(3, 128, 64, 1),
+         W4A16Config(3, 64, 64, 1), W4A16Config(3, 128, 64, 3)]
 
 
 @pytest.mark.parametrize("cfg", KIND3)
