@@ -92,11 +92,16 @@ __device__ __forceinline__ float term4(float4 a, float4 b) {
     return (a.x + a.y) + (a.z + a.w);
 }
 
-// VEC: x (and y) 16-byte aligned, float4 path with a scalar tail.
-template <bool DOT, bool VEC>
-__global__ __launch_bounds__(kRBlock) void sum_kernel(const float* __restrict__ x, const float* __restrict__ y, size_t n,
-                                                      float* __restrict__ out, unsigned* __restrict__ counter,
-                                                      float* __restrict__ partials) {
+// VEC: x (and y) 16-byte aligned, float4 path with a scalar tail.  BLOCK
+// threads per block: 256 for the streaming grid, 512 for small n (fewer blocks,
+// so fewer arrivals queue on the one ticket counter: those atomics serialise at
+// the counter's address, ~6 ns each, which at n = 1e6 with 245 blocks was
+// 1.5 of the kernel's 5.4 us).
+template <bool DOT, bool VEC, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void sum_kernel(const float* __restrict__ x, const float* __restrict__ y, size_t n,
+                                                    float* __restrict__ out, unsigned* __restrict__ counter,
+                                                    float* __restrict__ partials) {
+    constexpr int kRBlock = BLOCK;
     __shared__ float red[kRBlock / kWave];
     __shared__ int is_last;
     const size_t nthreads = (size_t)gridDim.x * kRBlock;
@@ -155,6 +160,12 @@ __global__ __launch_bounds__(kRBlock) void sum_kernel(const float* __restrict__ 
     }
 }
 
+// Inputs up to this many floats take 512-thread blocks, one unrolled step
+// per thread (n = 1e6: 123 blocks; 4.45 against 5.4 us from the first wave to
+// the result with 245 blocks of 256).
+constexpr size_t kSmallSum = (size_t)2 << 20;
+constexpr int kSmallBlock = 512;
+
 // Blocks for a sum over n floats.  Each thread's kRUnroll loads (2 kRUnroll
 // for a dot) are one grid-width apart, so the grid sets the stride between
 // the concurrent requests: 1.5 blocks per CU for the sum (1.5 MB apart on 256
@@ -162,7 +173,8 @@ __global__ __launch_bounds__(kRBlock) void sum_kernel(const float* __restrict__ 
 // to 1e9 floats, against 5.4-6.0 at 8 blocks per CU (8 MB apart: the
 // requests pile onto the same banks).
 unsigned sum_blocks(size_t n, bool dot) {
-    const size_t per_block = (size_t)kRBlock * 4 * kRUnroll;   // floats one block covers in one unrolled step
+    const int block = n <= kSmallSum ? kSmallBlock : kRBlock;
+    const size_t per_block = (size_t)block * 4 * kRUnroll;   // floats one block covers in one unrolled step
     size_t b = (n + per_block - 1) / per_block;
     const size_t cap = dot ? (size_t)cu_count() * 3 : (size_t)cu_count() * 3 / 2;
     if (b > cap) b = cap;
@@ -185,12 +197,21 @@ int launch_sum(const float* x, const float* y, size_t n, float* out, void* ws, s
         return fail(LTHIP_ERR_WORKSPACE, "%u blocks need %zu workspace bytes, %zu given", blocks,
                     256 + sizeof(float) * (size_t)blocks, ws_bytes);
     const bool vec = (reinterpret_cast<uintptr_t>(x) % 16 == 0) && (!DOT || reinterpret_cast<uintptr_t>(y) % 16 == 0);
-    if (vec)
-        hipLaunchKernelGGL((sum_kernel<DOT, true>), dim3(blocks), dim3(kRBlock), 0, as_stream(stream), x, y, n, out,
-                           counter, partials);
-    else
-        hipLaunchKernelGGL((sum_kernel<DOT, false>), dim3(blocks), dim3(kRBlock), 0, as_stream(stream), x, y, n, out,
-                           counter, partials);
+    hipStream_t s = as_stream(stream);
+    if (n <= kSmallSum) {
+        if (vec)
+            hipLaunchKernelGGL((sum_kernel<DOT, true, kSmallBlock>), dim3(blocks), dim3(kSmallBlock), 0, s, x, y, n,
+                               out, counter, partials);
+        else
+            hipLaunchKernelGGL((sum_kernel<DOT, false, kSmallBlock>), dim3(blocks), dim3(kSmallBlock), 0, s, x, y, n,
+                               out, counter, partials);
+    } else if (vec) {
+        hipLaunchKernelGGL((sum_kernel<DOT, true, kRBlock>), dim3(blocks), dim3(kRBlock), 0, s, x, y, n, out, counter,
+                           partials);
+    } else {
+        hipLaunchKernelGGL((sum_kernel<DOT, false, kRBlock>), dim3(blocks), dim3(kRBlock), 0, s, x, y, n, out, counter,
+                           partials);
+    }
     LT_TRY_LAUNCH(DOT ? "sdot launch" : "reduction launch");
     return LTHIP_OK;
 }
