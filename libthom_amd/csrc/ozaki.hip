@@ -23,6 +23,13 @@
 // kernel applies the same FP64 Horner recombination.  Other shapes use the
 // all-levels-in-registers kernel below.
 //
+// Non-finite inputs: a row of A (column of B) holding a NaN or an infinity has
+// no exponent to scale by.  The split marks it (exponent kOzNonFinite, all
+// digits 0), the combine steps leave its entries of C alone, and a fixup
+// kernel computes exactly those entries as a plain FP64 dot product, k
+// ascending, then alpha and beta as in the combine: NaN and infinities
+// propagate as in an FP64 product.  Other entries are unaffected.
+//
 // This routine has no counterpart in the reference; it sits beside the
 // native FP64 path (dgemm_final.cu) as the emulated alternative.
 #include "gemm_core2.hpp"
@@ -32,6 +39,7 @@ namespace {
 
 constexpr int kOzTile = 64;
 constexpr int kOzBK = 128;
+constexpr int32_t kOzNonFinite = INT32_MIN;   // split exponent of a row with a NaN or an infinity
 
 __device__ __forceinline__ uint32_t e4m3_of_digit(int d) {
     if (d == 0) return 0u;
@@ -48,23 +56,34 @@ __global__ __launch_bounds__(256) void ozaki_split_kernel(const double* __restri
                                                           uint8_t* __restrict__ digits, long slice_stride,
                                                           int32_t* __restrict__ exps) {
     __shared__ double red[4];
+    __shared__ int nonfinite[4];
     const int row = blockIdx.x;
     const double* xr = X + (long)row * ld;
     double mx = 0.0;
-    for (int c = threadIdx.x; c < cols; c += 256) mx = fmax(mx, fabs(xr[c]));
+    int nf = 0;
+    for (int c = threadIdx.x; c < cols; c += 256) {
+        const double v = xr[c];
+        nf |= !isfinite(v);
+        mx = fmax(mx, fabs(v));
+    }
     // block max (exact: max is order independent)
     for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    const bool wave_nf = __any(nf);
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = mx;
+        nonfinite[threadIdx.x >> 6] = wave_nf;
+    }
     __syncthreads();
     mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    const bool bad = nonfinite[0] | nonfinite[1] | nonfinite[2] | nonfinite[3];
     int e = 0;
-    if (mx > 0.0) frexp(mx, &e);
-    if (threadIdx.x == 0) exps[row] = e;
+    if (!bad && mx > 0.0) frexp(mx, &e);
+    if (threadIdx.x == 0) exps[row] = bad ? kOzNonFinite : e;
     uint8_t* drow = digits + (long)row * cols;
     for (int c4 = threadIdx.x * 4; c4 < cols; c4 += 1024) {
         double t[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) t[u] = ldexp(xr[c4 + u], -e);
+        for (int u = 0; u < 4; u++) t[u] = bad ? 0.0 : ldexp(xr[c4 + u], -e);
 #pragma unroll
         for (int p = 0; p < S; p++) {
             uint32_t word = 0;
@@ -77,6 +96,44 @@ __global__ __launch_bounds__(256) void ozaki_split_kernel(const double* __restri
             }
             *reinterpret_cast<uint32_t*>(drow + (long)p * slice_stride + c4) = word;
         }
+    }
+}
+
+// Entries of C whose row of A or column of B (row of Bt) holds a NaN or an
+// infinity: alpha * sum_k A[i][k] Bt[j][k], k ascending, (+ beta C).  One
+// 64 x 64 tile of C per block; a tile with no such row or column returns at
+// once.
+__global__ __launch_bounds__(256) void ozaki_nonfinite_kernel(const double* __restrict__ A, const double* __restrict__ Bt,
+                                                              const int32_t* __restrict__ ea,
+                                                              const int32_t* __restrict__ eb, double* __restrict__ C,
+                                                              int M, int N, int K, double alpha, double beta) {
+    __shared__ int flag_r[64], flag_c[64];
+    __shared__ int any;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int r = m0 + threadIdx.x;
+        flag_r[threadIdx.x] = r < M && ea[r] == kOzNonFinite;
+        if (flag_r[threadIdx.x]) any = 1;
+    } else if (threadIdx.x < 128) {
+        const int c = n0 + threadIdx.x - 64;
+        flag_c[threadIdx.x - 64] = c < N && eb[c] == kOzNonFinite;
+        if (flag_c[threadIdx.x - 64]) any = 1;
+    }
+    __syncthreads();
+    if (!any) return;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int i = e / 64, j = e % 64, row = m0 + i, col = n0 + j;
+        if (row >= M || col >= N || !(flag_r[i] | flag_c[j])) continue;
+        const double* a = A + (long)row * K;
+        const double* b = Bt + (long)col * K;
+        double dot = 0.0;
+        for (int k = 0; k < K; k++) dot = __dadd_rn(dot, __dmul_rn(a[k], b[k]));
+        double v = __dmul_rn(alpha, dot);
+        double* cp = C + (long)row * N + col;
+        if (beta != 0.0) v = __dadd_rn(v, __dmul_rn(beta, *cp));
+        *cp = v;
     }
 }
 
@@ -189,7 +246,7 @@ __global__ __launch_bounds__(256) void ozaki_fp8_kernel(const uint8_t* __restric
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = row0 + wm + 16 * i + 4 * fk + r, col = col0 + wn + 16 * j + fr;
-                if (row >= M || col >= N) continue;
+                if (row >= M || col >= N || ea[row] == kOzNonFinite || eb[col] == kOzNonFinite) continue;
                 double h = (double)acc[S - 1][i][j][r];
 #pragma unroll
                 for (int l = S - 2; l >= 0; l--) h = __dadd_rn((double)acc[l][i][j][r], __dmul_rn(h, 0x1p-5));
@@ -348,6 +405,7 @@ __global__ __launch_bounds__(256) void ozaki_combine_kernel(const float* __restr
     const long total = (long)M * N, slice = total;
     for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
         const int row = (int)(e / N), col = (int)(e % N);
+        if (ea[row] == kOzNonFinite || eb[col] == kOzNonFinite) continue;   // the fixup kernel's entries
         double h = (double)lv[(long)(S - 1) * slice + e];
         for (int l = S - 2; l >= 0; l--) h = __dadd_rn((double)lv[(long)l * slice + e], __dmul_rn(h, 0x1p-5));
         double v = __dmul_rn(alpha, ldexp(h, ea[row] + eb[col] - 8));
@@ -507,12 +565,17 @@ int lthip_dgemm_ozaki(const double* A, const double* B, double* C, size_t M, siz
         hipLaunchKernelGGL(ozaki_combine_kernel, dim3((unsigned)blocks), dim3(256), 0, s, lv, ea, eb, C, m, n, slices,
                            alpha, beta);
         LT_TRY_LAUNCH("ozaki combine launch");
-        return LTHIP_OK;
+    } else {
+        const int tiles_m = (m + kOzTile - 1) / kOzTile, tiles_n = (n + kOzTile - 1) / kOzTile;
+        hipLaunchKernelGGL(kProd[slices], dim3((unsigned)(tiles_m * tiles_n)), dim3(256), 0, s, ad, bd, ea, eb, C, m,
+                           n, k, alpha, beta, tiles_m, tiles_n);
+        LT_TRY_LAUNCH("ozaki product launch");
     }
-    const int tiles_m = (m + kOzTile - 1) / kOzTile, tiles_n = (n + kOzTile - 1) / kOzTile;
-    hipLaunchKernelGGL(kProd[slices], dim3((unsigned)(tiles_m * tiles_n)), dim3(256), 0, s, ad, bd, ea, eb, C, m, n, k,
-                       alpha, beta, tiles_m, tiles_n);
-    LT_TRY_LAUNCH("ozaki product launch");
+    if (k > 0) {   // entries whose row or column holds a NaN or an infinity
+        hipLaunchKernelGGL(ozaki_nonfinite_kernel, dim3((unsigned)((n + 63) / 64), (unsigned)((m + 63) / 64)), dim3(256),
+                           0, s, A, bt, ea, eb, C, m, n, k, alpha, beta);
+        LT_TRY_LAUNCH("ozaki non-finite fixup launch");
+    }
     return LTHIP_OK;
 }
 

@@ -276,12 +276,16 @@ void orc_ozaki_split_rows(const double *X, int rows, int cols, int ld, int slice
     for (int r = 0; r < rows; r++) {
         const double *xr = X + (size_t)r * ld;
         double mx = 0.0;
-        for (int c = 0; c < cols; c++) mx = fmax(mx, fabs(xr[c]));
-        int e = 0;
-        if (mx > 0.0) frexp(mx, &e);          /* mx = f 2^e, f in [0.5, 1) */
-        exps[r] = e;
+        int bad = 0;
         for (int c = 0; c < cols; c++) {
-            double t = ldexp(xr[c], -e);      /* |t| < 1 */
+            bad |= !isfinite(xr[c]);
+            mx = fmax(mx, fabs(xr[c]));
+        }
+        int e = 0;
+        if (!bad && mx > 0.0) frexp(mx, &e);  /* mx = f 2^e, f in [0.5, 1) */
+        exps[r] = bad ? ORC_OZAKI_NONFINITE : e;
+        for (int c = 0; c < cols; c++) {
+            double t = bad ? 0.0 : ldexp(xr[c], -e);   /* |t| < 1 */
             for (int p = 0; p < slices; p++) {
                 double v = ldexp(t, p == 0 ? 4 : 5);   /* 16 t, then 32 r */
                 double d = nearbyint(v);
@@ -294,6 +298,7 @@ void orc_ozaki_split_rows(const double *X, int rows, int cols, int ld, int slice
 
 typedef struct {
     const int8_t *da, *db;   /* [s][M][K], [s][N][K] */
+    const double *A, *Bt;    /* for entries with a non-finite row or column */
     const int32_t *ea, *eb;
     double *C;
     int M, N, K, s;
@@ -306,6 +311,16 @@ static void ozaki_rows(void *p, int lo, int hi) {
     int64_t S[64];
     for (int i = lo; i < hi; i++) {
         for (int j = 0; j < a->N; j++) {
+            if (a->ea[i] == ORC_OZAKI_NONFINITE || a->eb[j] == ORC_OZAKI_NONFINITE) {
+                /* a plain FP64 dot product, k ascending, then alpha and beta */
+                double dot = 0.0;
+                for (int k = 0; k < a->K; k++) dot = dot + a->A[(size_t)i * a->K + k] * a->Bt[(size_t)j * a->K + k];
+                double *c = a->C + (size_t)i * a->N + j;
+                double out = a->alpha * dot;
+                if (a->beta != 0.0) out = out + a->beta * *c;
+                *c = out;
+                continue;
+            }
             for (int L = 2; L <= s + 1; L++) S[L] = 0;
             for (int pa = 1; pa <= s; pa++) {
                 const int8_t *ra = a->da + ((size_t)(pa - 1) * a->M + i) * a->K;
@@ -338,7 +353,7 @@ void orc_ozaki_dgemm(const double *A, const double *B, double *C, int M, int N, 
         for (int j = 0; j < N; j++) Bt[(size_t)j * K + k] = B[(size_t)k * N + j];
     orc_ozaki_split_rows(A, M, K, K, slices, da, ea);
     orc_ozaki_split_rows(Bt, N, K, K, slices, db, eb);   /* columns of B */
-    ozaki_args a = {da, db, ea, eb, C, M, N, K, slices, alpha, beta};
+    ozaki_args a = {da, db, A, Bt, ea, eb, C, M, N, K, slices, alpha, beta};
     parallel_for(M, threads, ozaki_rows, &a);
     free(da); free(db); free(ea); free(eb); free(Bt);
 }

@@ -73,13 +73,18 @@ void orc_dgemm_dot2_entries(const double *A, const double *B, int M, int N, int 
 /* (new) Ozaki split of the rows of X (ld = row stride): per row r an exponent
  * e_r with max|X[r,:]| = f 2^e_r, f in [0.5,1) (0 for a zero row), and
  * digits d_1 = rint(16 t), d_p = rint(32 r_{p-1}) with t = X 2^-e_r,
- * r_p = (previous) - d_p.  digits[p][r][c] in [-16,16]. */
+ * r_p = (previous) - d_p.  digits[p][r][c] in [-16,16].  A row holding a NaN
+ * or an infinity gets e_r = ORC_OZAKI_NONFINITE and all-zero digits. */
+#define ORC_OZAKI_NONFINITE ((int32_t)0x80000000)
 void orc_ozaki_split_rows(const double *X, int rows, int cols, int ld, int slices,
                           int8_t *digits, int32_t *exps);
 /* (new) emulated dgemm with exactly the GPU's combination order:
  * S_L = sum_{p+q=L} D^A_p (D^B_q)^T  (exact integers), acc = S_{s+1};
  * acc = S_L + acc*2^-5 for L = s..2;  r = ldexp(acc, e_i + f_j - 8);
- * C = alpha*r + beta*C (beta term only if beta != 0).  No FMA contraction. */
+ * C = alpha*r + beta*C (beta term only if beta != 0).  No FMA contraction.
+ * Entries whose row of A or column of B holds a NaN or an infinity:
+ * alpha * (sum_k A_ik B_kj, k ascending) (+ beta*C) in plain FP64, so NaN and
+ * infinities propagate as in an FP64 product. */
 void orc_ozaki_dgemm(const double *A, const double *B, double *C, int M, int N, int K,
                      double alpha, double beta, int slices, int threads);
 

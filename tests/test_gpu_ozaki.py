@@ -104,3 +104,26 @@ def test_config_error_vs_slices(dev, scaled):
     assert errs[6] < errs[4] / 100 and errs[8] < errs[6] / 100
     # 4 + 5 (S - 1) bits of each operand relative to its row/column maximum
     assert errs[8] < 2.0 ** -34
+
+
+@pytest.mark.parametrize("M,N,K", [(100, 70, 256), (256, 128, 256)])
+def test_ozaki_nonfinite_inputs(dev, orc, M, N, K):
+    # NaN in a row of A, +inf and -inf in columns of B, a 0 * inf entry: the
+    # marked rows/columns follow the FP64 product, bit for bit with the oracle
+    # (NaN in the same places), on both the all-levels and the level-GEMM path
+    rng = np.random.default_rng(M + N)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    A[3, 5] = np.nan
+    B[7, 2] = np.inf
+    B[1, 4] = -np.inf
+    A[6, 1] = 0.0
+    C0 = rng.standard_normal((M, N))
+    got = R.dgemm_ozaki(t(A, dev), t(B, dev), 6, t(C0, dev), 1.5, -0.5).cpu().numpy()
+    ref = orc.ozaki_dgemm(A, B, 6, C0, 1.5, -0.5)
+    assert np.array_equal(got, ref, equal_nan=True)
+    assert np.isnan(got[3]).all() and np.isnan(got[6, 4]) and np.isinf(got[0, 2])
+    keep = np.ones((M, N), bool)
+    keep[3, :] = False
+    keep[:, [2, 4]] = False
+    assert np.isfinite(got[keep]).all()

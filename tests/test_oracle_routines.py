@@ -206,3 +206,41 @@ def test_mlp_gradient_finite_differences(orc):
             lm, _ = orc.mlp_loss_grad(x, *params, t, mask)
             p[idx] = old
             assert g[idx] == pytest.approx((lp - lm) / (2 * eps), rel=1e-5, abs=1e-7)
+
+
+def _nonfinite_inputs(M=9, N=7, K=32, seed=4):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    A[3, 5] = np.nan            # row 3 of A
+    B[7, 2] = np.inf            # column 2 of B
+    B[1, 4] = -np.inf           # column 4 of B
+    A[6, 1] = 0.0               # 0 * (-inf) in entry (6, 4): NaN
+    return A, B
+
+
+def test_ozaki_nonfinite_rows_and_columns(orc):
+    # a row or column holding a NaN or an infinity: split exponent marked, digits
+    # zero; its entries follow the FP64 product (NaN and infinities propagate,
+    # compared with an exact float64 evaluation); every other entry equals the
+    # Ozaki product of the same inputs with the bad rows/columns made finite
+    A, B = _nonfinite_inputs()
+    d, e = orc.ozaki_split_rows(A, 5)
+    assert e[3] == -2 ** 31 and not d[:, 3].any() and (e[np.arange(9) != 3] != -2 ** 31).all()
+    C0 = np.random.default_rng(1).standard_normal((9, 7))
+    got = orc.ozaki_dgemm(A, B, 5, C0, 1.5, -0.5)
+    with np.errstate(invalid="ignore"):
+        plain = 1.5 * np.array([[sum(A[i, k] * B[k, j] for k in range(32)) for j in range(7)] for i in range(9)]) \
+            - 0.5 * C0
+    bad = np.zeros((9, 7), bool)
+    bad[3, :] = True
+    bad[:, [2, 4]] = True
+    assert np.isnan(got[3]).all()
+    assert np.array_equal(np.isnan(got[bad]), np.isnan(plain[bad]))
+    assert np.array_equal(got[bad & ~np.isnan(got)], plain[bad & ~np.isnan(plain)])   # the infinities
+    assert np.isnan(got[6, 4]) and np.isinf(got[0, 2])   # 0 * -inf; a finite row against +inf
+    Af, Bf = A.copy(), B.copy()
+    Af[3] = 1.0
+    Bf[:, [2, 4]] = 1.0
+    fin = orc.ozaki_dgemm(Af, Bf, 5, C0, 1.5, -0.5)
+    assert np.array_equal(got[~bad], fin[~bad]) and np.isfinite(got[~bad]).all()
