@@ -55,7 +55,8 @@ W4A16_SPACE: List[Tuple[int, int, int, int]] = (
     [(2, 128, 128, s) for s in (1, 2, 4)] +
     [(2, bn, bm, 1) for bm, bn in ((128, 256), (256, 128), (256, 256))] +
     [(3, 128, 128, 1), (3, 128, 128, 2), (3, 256, 128, 1), (3, 128, 256, 1), (3, 256, 256, 1)] +
-    [(4, 64, 0, 1), (4, 32, 0, 1), (5, 64, 0, 1)])
+    [(4, 64, 0, 1), (4, 32, 0, 1), (5, 64, 0, 1)] +
+    [(3, 128, 64, 1), (3, 128, 64, 2), (3, 64, 64, 1), (5, 32, 0, 1)])
 
 
 def valid(cfg: Tuple[int, int, int, int], M: int, K: int, N: Optional[int] = None,

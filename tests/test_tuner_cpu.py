@@ -24,12 +24,14 @@ def test_db_roundtrip(tmp_path):
 
 
 def test_space_and_validity():
-    assert len(tuner.W4A16_SPACE) == 30
+    assert len(tuner.W4A16_SPACE) == 34
     assert tuner.valid((4, 64, 0, 1), 4, 4096, 11008, 32) and not tuner.valid((4, 64, 0, 1), 5, 4096, 11008, 128)
     assert tuner.valid((4, 32, 0, 1), 2, 4096, 4096, 64) and not tuner.valid((4, 32, 0, 1), 2, 4096, 4096, 32)
     assert not tuner.valid((4, 64, 0, 1), 4, 16384, 4096, 128)
     assert tuner.valid((5, 64, 0, 1), 64, 4096, 520, 32) and not tuner.valid((5, 64, 0, 1), 65, 4096, 520, 32)
     assert not tuner.valid((5, 64, 0, 1), 16, 4096)
+    assert tuner.valid((5, 32, 0, 1), 64, 4096, 520, 32) and not tuner.valid((5, 32, 0, 1), 65, 4096, 520, 32)
+    assert tuner.valid((3, 128, 64, 1), 200, 4096, 11008, 128) and not tuner.valid((3, 128, 64, 1), 200, 4096, 520, 128)
     assert tuner.valid((1, 64, 16, 8), 128, 4096) and not tuner.valid((1, 64, 16, 8), 129, 4096)
     assert tuner.valid((2, 256, 256, 1), 1, 512)
     # kind 3: N % 32 == 0 and group % 64 == 0, known only with N and group
