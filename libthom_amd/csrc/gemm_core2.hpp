@@ -5,21 +5,29 @@
 // ring of STAGES LDS buffers: while slab s is multiplied, slabs s+1 .. s+STAGES-1
 // are in flight.  One barrier per slab.
 //
+// The product is built from v_mfma_f32_16x16x32_f16 (16 x 16 outputs, 32 k per
+// instruction): it sustains a higher clock than the 32 x 32 x 16 shape at equal
+// flops (half the accumulator bytes per flop; measured with MFMA-only loops:
+// about 2.0 against 1.7 GHz under the board power limit), which is worth more
+// than its extra instructions.
+// Operand layout: lane l holds A[l % 16][8 (l / 16) .. +7] and
+// B[8 (l / 16) .. +7][l % 16]; it receives C[4 (l / 16) .. +3][l % 16].
+//
 //   A tile [BM][BK] fp16, K contiguous; 16-byte chunk kc of row r sits at
-//   kc ^ swzA(r).  MFMA A fragments (8 consecutive k of one row) are ds_read_b128.
+//   kc ^ swzA(r).  An A fragment (8 consecutive k of one row) is one ds_read_b128.
 //   B tile [BK][BN] fp16, N contiguous (as in memory); 16-byte chunk nc of row k
-//   sits at nc ^ 4(k & 3).  MFMA B fragments (8 consecutive k of one column) are
-//   two ds_read_b64_tr_b16.  That instruction transposes within each 16-lane
-//   group: treating lane s's four halves as row s/4, columns 4(s%4)..+3 of a
-//   4 x 16 block, lane i receives column i.  So lane s addresses row s/4 and
-//   column 4(s%4) of the 4 x 16 block that its group needs.
+//   sits at nc ^ swzB(k).  A B fragment (8 consecutive k of one column) is two
+//   ds_read_b64_tr_b16.  That instruction transposes within each 16-lane group:
+//   treating lane s's four halves as row s/4, columns 4(s%4)..+3 of a 4 x 16
+//   block, lane i receives column i.  So lane s addresses row s/4 and column
+//   4(s%4) of the 4 x 16 block (k rows 8g..8g+3, then 8g+4..8g+7 for group g).
 //
 // The swizzles keep the fragment reads free of LDS bank conflicts (below); the DMA
 // fills are contiguous by construction, and the lane that fills an LDS chunk
 // fetches the global chunk that belongs there, so the swizzle costs nothing.
 //
 // Block = WGM x WGN waves, each owning a (BM/WGM) x (BN/WGN) sub-tile of
-// v_mfma_f32_32x32x16_f16 tiles with FP32 accumulators.
+// 16 x 16 MFMA tiles with FP32 accumulators.
 //
 // Requirements (checked by the caller): K % 8 == 0, N % 8 == 0, lda and ldb
 // multiples of 8, A and B 16-byte aligned.  Rows past M and columns past N are
@@ -40,14 +48,15 @@ typedef __attribute__((address_space(3))) char lds_char_t;
 // (256 bytes) at a time, a ds_read_b64_tr_b16 32 lanes at a time.
 //  A: 16 consecutive rows, same logical chunk -> 16 distinct 16-byte slots
 //     of a 256-byte bank row (rows are 128 or 64 bytes long).
-//  B: rows k..k+3 x 32 columns (4 chunks each) -> 16 distinct slots (rows are
-//     a multiple of 256 bytes long).
+//  B: two lane groups read k rows {k0..k0+3} and {k0+8..k0+11} x 16 columns
+//     (2 chunks) -> the 8 rows differ in (k & 3) or bit 3 of k, so chunk ^ swzB
+//     gives 16 distinct slots (rows are a multiple of 256 bytes long).
 template <int BK>
 __device__ __forceinline__ int swzA2(int row) {
     if constexpr (BK == 64) return (row >> 1) & 7;
     else return (row >> 2) & 3;
 }
-__device__ __forceinline__ int swzB2(int k) { return 4 * (k & 3); }
+__device__ __forceinline__ int swzB2(int k) { return (4 * (k & 3)) ^ (2 * ((k >> 3) & 1)); }
 
 // s_waitcnt with vmcnt <= n, other counters untouched (gfx9 encoding)
 template <int n>
@@ -70,7 +79,7 @@ template <int BM_, int BN_, int BK_, int STAGES_, int WGM_, int WGN_>
 struct Core2 {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, STAGES = STAGES_, WGM = WGM_, WGN = WGN_;
     static constexpr int WAVES = WGM * WGN, THREADS = 64 * WAVES;
-    static constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 32, TN = WN / 32;
+    static constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 16, TN = WN / 16;
     static constexpr int A_BYTES = BM * BK * 2, B_BYTES = BK * BN * 2;
     static constexpr int STAGE_BYTES = A_BYTES + B_BYTES;
     static constexpr int A_INSTR = A_BYTES / (1024 * WAVES);   // 1 KiB DMA instructions per wave per slab
@@ -79,10 +88,9 @@ struct Core2 {
     static constexpr int KC = BK / 8, NC = BN / 8;
     static constexpr int A_ROWS_PER = 1024 / (BK * 2), B_ROWS_PER = 1024 / (BN * 2);
     static constexpr int SMEM = STAGES * STAGE_BYTES;
-    static_assert(TM >= 1 && TN >= 1 && WM % 32 == 0 && WN % 32 == 0, "wave tile");
+    static_assert(TM >= 1 && TN >= 1 && WM % 16 == 0 && WN % 16 == 0, "wave tile");
     static_assert(A_BYTES % (1024 * WAVES) == 0 && B_BYTES % (1024 * WAVES) == 0, "tile too small for the DMA split");
     static_assert(BK == 32 || BK == 64, "BK");
-    static_assert(TM % 2 == 0, "the last k-step splits the M tiles in halves");
     static_assert(BN % 128 == 0 && BN * 2 <= 1024, "BN");
     static_assert(STAGES >= 3 && SMEM <= 160 * 1024, "LDS budget (the ring needs 3 stages)");
     static_assert(LOADS * (STAGES - 1) < 64, "vmcnt range");
@@ -169,32 +177,40 @@ struct Core2Src {
     }
 };
 
+// Core2Src::issue as a free function, for the calls from inside generic
+// lambdas (the member-template call fails to resolve there in the host pass)
+template <int Q0, int Q1, class S>
+__device__ __forceinline__ void core2_issue(const S& src, char* smem, int stage, int slab) {
+    src.template issue<Q0, Q1>(smem, stage, slab);
+}
+
 template <class C>
 struct Core2Frags {
     v8h a[C::TM], b[C::TN];
 
-    __device__ __forceinline__ void load(const char* sa, const char* sb, int kk, int wm, int wn) {
+    // fragment F of k-step kk (32 k): F < TM is A fragment F, else B fragment F - TM
+    template <int F>
+    __device__ __forceinline__ void load_one(const char* sa, const char* sb, int kk, int wm, int wn) {
         const int lane = threadIdx.x & 63;
-        const int frow = lane & 31, fkc = lane >> 5;   // A: row within a 32-row tile, 8-k chunk within 16
-        const int g = lane >> 4, gi = lane & 15;       // B: 16-lane group, lane within it
-        const int bcol = 16 * (g & 1) + 4 * (gi & 3);  // first column of this lane's 8-byte read
-        const int brow = 8 * (g >> 1) + (gi >> 2);     // k row of the first read (+4 for the second)
-#pragma unroll
-        for (int i = 0; i < C::TM; i++) {
-            const int row = wm + 32 * i + frow;
-            const int kc = (2 * kk + fkc) ^ swzA2<C::BK>(row);
-            a[i] = *reinterpret_cast<const v8h*>(sa + row * (C::BK * 2) + kc * 16);
-        }
-#pragma unroll
-        for (int j = 0; j < C::TN; j++) {
-            const int col = wn + 32 * j + bcol;
-            const int k1 = 16 * kk + brow, k2 = k1 + 4;
+        const int g = lane >> 4, gi = lane & 15;   // 16-lane group: k 8g..8g+7
+        if constexpr (F < C::TM) {
+            const int row = wm + 16 * F + gi;
+            const int kc = (4 * kk + g) ^ swzA2<C::BK>(row);
+            a[F] = *reinterpret_cast<const v8h*>(sa + row * (C::BK * 2) + kc * 16);
+        } else {
+            constexpr int j = F - C::TM;
+            const int col = wn + 16 * j + 4 * (gi & 3);
+            const int k1 = 32 * kk + 8 * g + (gi >> 2), k2 = k1 + 4;
             const int off1 = k1 * (C::BN * 2) + (((col >> 3) ^ swzB2(k1)) << 4) + (col & 7) * 2;
             const int off2 = k2 * (C::BN * 2) + (((col >> 3) ^ swzB2(k2)) << 4) + (col & 7) * 2;
             const v4s lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s_t*)(lds_char_t*)(sb + off1));
             const v4s hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s_t*)(lds_char_t*)(sb + off2));
             b[j] = __builtin_bit_cast(v8h, (v8s)__builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
         }
+    }
+
+    __device__ __forceinline__ void load(const char* sa, const char* sb, int kk, int wm, int wn) {
+        static_for<0, C::TM + C::TN>([&](auto fc) { load_one<decltype(fc)::value>(sa, sb, kk, wm, wn); });
     }
 };
 
@@ -235,23 +251,40 @@ __device__ __forceinline__ void core2_issue_clamped(char* smem, const __half* __
 // (the last slab of the matrix may be partial: K % 8 == 0 suffices); with
 // splits > 1 the partial sums go through out.partial and the last split to
 // finish sums them in split order (same protocol as gemm_tile).
+// Positions within a k-step of MF MFMAs: fragment load f (of NF) goes after
+// MFMA floor(f MF / NF), DMA instruction d (of ND) after MFMA
+// 1 + floor(d MF / ND), so both spread evenly and the DMA sits between reads.
+template <int NF, int MF>
+constexpr int frag_at(int q) {
+    for (int f = 0; f < NF; f++)
+        if ((f * MF) / NF == q) return f;
+    return -1;
+}
+template <int ND, int MF>
+constexpr int dma_at(int q) {
+    for (int d = 0; d < ND; d++)
+        if ((1 + (d * MF) / ND) % MF == q) return d;
+    return -1;
+}
+
 template <class C>
 __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict__ A, long lda,
                                            const __half* __restrict__ B, long ldb, int M, int N, int K, int m0,
                                            int n0, int s0, int s1, int split, int splits, const TileOut& out) {
-    constexpr int TM = C::TM, TN = C::TN, BK = C::BK, STAGES = C::STAGES, KSTEPS = BK / 16;
-    constexpr int MFMAS = TM * TN, READS = TM + 2 * TN;
+    constexpr int TM = C::TM, TN = C::TN, BK = C::BK, STAGES = C::STAGES, KSTEPS = BK / 32;
+    constexpr int MFMAS = TM * TN, NF = TM + TN;   // per k-step: MFMAs, fragment loads
+    constexpr int D = C::LOADS / KSTEPS;            // DMA instructions per k-step
+    static_assert(C::LOADS % KSTEPS == 0, "DMA chunks");
+    static_assert(NF <= MFMAS && D <= MFMAS, "at most one fragment load and one DMA instruction per MFMA");
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = (wave / C::WGN) * C::WM, wn = (wave % C::WGN) * C::WN;
 
-    v16f acc[TM][TN];
+    v4f acc[TM][TN];
 #pragma unroll
     for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < TN; j++)
-#pragma unroll
-            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
+        for (int j = 0; j < TN; j++) acc[i][j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
 
     // full slabs go through the ring; a partial last slab (K % BK != 0) is
     // multiplied afterwards with the k beyond K masked out
@@ -262,13 +295,14 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
         Core2Src<C> src;
         src.init(A, lda, B, ldb, M, N, m0, n0, s0 * BK);
         wait_vmcnt<0>();   // nothing older may be counted against the ring below
-        // Ring of STAGES slabs.  While slab s is multiplied, slab s+STAGES-1 is
-        // fetched into the stage slab s-1 used, its DMA instructions spread over
-        // the k-steps of slab s (chunk kk during k-step kk).  The hand-over in the
-        // last k-step waits for slab s+1, with slabs s+2 .. s+STAGES-1 (all issued
-        // by then) still in flight.  Past the end the last slab is fetched again
-        // into a stage nobody reads, so the vmcnt arithmetic never changes.
-        // DMA chunk kk = instructions [kk * LOADS / KSTEPS, (kk + 1) * LOADS / KSTEPS)
+        // Ring of STAGES slabs.  During slab s, slab s+STAGES-1 is fetched into the
+        // stage slab s-1 used (chunk kk of its DMA during k-step kk).  Every k-step
+        // multiplies fragments loaded during the k-step before and loads the next
+        // k-step's fragments; before the first read of a new stage (at the start of
+        // the last k-step of each slab) a barrier makes sure that slab landed for
+        // every wave, and that every wave is done with the stage the next DMA
+        // overwrites.  Past the end the last slab is fetched again into a stage
+        // nobody reads, so the vmcnt arithmetic never changes.
 #pragma unroll
         for (int p = 0; p < STAGES - 1; p++) src.template issue<0, C::LOADS>(smem, p, p < nslabs ? p : nslabs - 1);
         ring_barrier<C::LOADS * (STAGES - 2)>();
@@ -277,52 +311,52 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
         fr[0].load(smem, smem + C::A_BYTES, 0, wm, wn);
         int stage = 0;                  // s % STAGES
         int fill_stage = STAGES - 1;    // (s + STAGES - 1) % STAGES
-        for (int s = 0; s < nslabs; s++) {
-            const char* sa = smem + stage * C::STAGE_BYTES;
-            const char* sb = sa + C::A_BYTES;
-            const int fill = s + STAGES - 1 < nslabs ? s + STAGES - 1 : nslabs - 1;
+        // one k-step on fr[CUR], loading fr[CUR ^ 1] from (la, lb, lk) and
+        // issuing DMA chunk KK of slab `fill`; the order of MFMAs, fragment loads
+        // and DMA instructions is fixed by a scheduling fence after each MFMA
+        auto kstep = [&](auto curc, auto kkc, const char* la, const char* lb, int lk, int fill) {
+            constexpr int CUR = decltype(curc)::value, KK = decltype(kkc)::value;
+            if constexpr (KK == KSTEPS - 1)   // slab s+1 landed, newest DMA: chunks 0..KK-1 of slab s+STAGES-1
+                ring_barrier<C::LOADS * (STAGES - 3) + KK * D>();
             __builtin_amdgcn_sched_barrier(0);
-            // k-steps 0 .. KSTEPS-2: MFMAs of step kk, fragment reads of step
-            // kk+1 and DMA chunk kk; each step is its own scheduling region
-            static_for<0, KSTEPS - 1>([&](auto kkc) {
-                constexpr int kk = decltype(kkc)::value;
-                fr[(kk + 1) & 1].load(sa, sb, kk + 1, wm, wn);
-                src.template issue<kk * C::LOADS / KSTEPS, (kk + 1) * C::LOADS / KSTEPS>(smem, fill_stage, fill);
-                const Core2Frags<C>& f = fr[kk & 1];
-#pragma unroll
-                for (int i = 0; i < TM; i++)
-#pragma unroll
-                    for (int j = 0; j < TN; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
-                interleave<MFMAS, READS, (kk + 1) * C::LOADS / KSTEPS - kk * C::LOADS / KSTEPS>();
+            static_for<0, MFMAS>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                constexpr int i = q / TN, j = q % TN;
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fr[CUR].a[i], fr[CUR].b[j], acc[i][j], 0, 0, 0);
+                constexpr int f = frag_at<NF, MFMAS>(q);
+                if constexpr (f >= 0) fr[CUR ^ 1].template load_one<f>(la, lb, lk, wm, wn);
+                constexpr int dd = dma_at<D, MFMAS>(q);
+                if constexpr (dd >= 0) {
+                    constexpr int d = KK * D + dd;
+                    core2_issue<d, d + 1>(src, smem, fill_stage, fill);
+                }
                 __builtin_amdgcn_sched_barrier(0);
             });
-            // last k-step: half the MFMAs with the last DMA chunk, hand-over,
-            // first fragments of slab s+1 with the other half
-            {
-                constexpr int KL = KSTEPS - 1;
-                const Core2Frags<C>& f = fr[KL & 1];
-                src.template issue<KL * C::LOADS / KSTEPS, C::LOADS>(smem, fill_stage, fill);
-#pragma unroll
-                for (int i = 0; i < TM / 2; i++)
-#pragma unroll
-                    for (int j = 0; j < TN; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
-                interleave<MFMAS / 2, 0, C::LOADS - KL * C::LOADS / KSTEPS>();
-                __builtin_amdgcn_sched_barrier(0);
-                ring_barrier<C::LOADS * (STAGES - 2)>();
-                stage = stage + 1 == STAGES ? 0 : stage + 1;
-                fill_stage = fill_stage + 1 == STAGES ? 0 : fill_stage + 1;
-                const char* na = smem + stage * C::STAGE_BYTES;
-                fr[0].load(na, na + C::A_BYTES, 0, wm, wn);   // KSTEPS is even
-#pragma unroll
-                for (int i = TM / 2; i < TM; i++)
-#pragma unroll
-                    for (int j = 0; j < TN; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
-                interleave<MFMAS / 2, READS, 0>();
-                __builtin_amdgcn_sched_barrier(0);
+        };
+        auto slab = [&](auto c0, int s) {
+            constexpr int C0 = decltype(c0)::value;   // fragment buffer of the slab's first k-step
+            const char* sa = smem + stage * C::STAGE_BYTES;
+            const int fill = s + STAGES - 1 < nslabs ? s + STAGES - 1 : nslabs - 1;
+            const int nstage = stage + 1 == STAGES ? 0 : stage + 1;
+            const char* na = smem + nstage * C::STAGE_BYTES;
+            if constexpr (KSTEPS == 1) {
+                kstep(int_c<C0>{}, int_c<0>{}, na, na + C::A_BYTES, 0, fill);
+            } else {
+                kstep(int_c<C0>{}, int_c<0>{}, sa, sa + C::A_BYTES, 1, fill);
+                kstep(int_c<C0 ^ 1>{}, int_c<1>{}, na, na + C::A_BYTES, 0, fill);
             }
+            stage = nstage;
+            fill_stage = fill_stage + 1 == STAGES ? 0 : fill_stage + 1;
+        };
+        if constexpr (KSTEPS == 2) {
+            for (int s = 0; s < nslabs; s++) slab(int_c<0>{}, s);
+        } else {
+            int s = 0;
+            for (; s + 1 < nslabs; s += 2) {
+                slab(int_c<0>{}, s);
+                slab(int_c<1>{}, s + 1);
+            }
+            if (s < nslabs) slab(int_c<0>{}, s);
         }
         wait_vmcnt<0>();   // the redundant tail DMA must land before the LDS is reused
     }
@@ -331,12 +365,12 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
         ring_barrier<0>();   // every wave is done with the ring
         core2_issue_clamped<C>(smem, A, lda, B, ldb, M, N, K, m0, n0, k0);
         ring_barrier<0>();
-        const int kl = 8 * (lane >> 5);   // first k of this lane's 8 in a k-step
+        const int kl = 8 * (lane >> 4);   // first k of this lane's 8 in a k-step
 #pragma unroll
         for (int kk = 0; kk < KSTEPS; kk++) {
             Core2Frags<C> f;
             f.load(smem, smem + C::A_BYTES, kk, wm, wn);
-            if (16 * kk + kl >= rem) {
+            if (32 * kk + kl >= rem) {
 #pragma unroll
                 for (int i = 0; i < TM; i++) f.a[i] = v8h{};
 #pragma unroll
@@ -346,21 +380,21 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
             for (int i = 0; i < TM; i++)
 #pragma unroll
                 for (int j = 0; j < TN; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
         }
     }
     __syncthreads();   // the caller may refill the ring for its next tile
 
-    const int fcol = lane & 31, frow4 = 4 * (lane >> 5);
+    const int fcol = lane & 15, frow4 = 4 * (lane >> 4);
     if (splits <= 1) {
 #pragma unroll
         for (int i = 0; i < TM; i++)
 #pragma unroll
             for (int j = 0; j < TN; j++)
 #pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    const int row = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
-                    const int col = n0 + wn + 32 * j + fcol;
+                for (int r = 0; r < 4; r++) {
+                    const int row = m0 + wm + 16 * i + frow4 + r;
+                    const int col = n0 + wn + 16 * j + fcol;
                     if (row < M && col < N) {
                         __half* cp = out.C + (long)row * out.ldc + col;
                         float v = out.alpha * acc[i][j][r];
@@ -376,9 +410,9 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
 #pragma unroll
         for (int j = 0; j < TN; j++)
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int row = m0 + wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3);
-                const int col = n0 + wn + 32 * j + fcol;
+            for (int r = 0; r < 4; r++) {
+                const int row = m0 + wm + 16 * i + frow4 + r;
+                const int col = n0 + wn + 16 * j + fcol;
                 if (row < M && col < N) publish(mine + (long)row * N + col, acc[i][j][r]);
             }
     int* last_split = reinterpret_cast<int*>(smem);   // the ring is idle by now

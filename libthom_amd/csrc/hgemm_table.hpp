@@ -15,8 +15,8 @@
 // reach LDS by DMA through a ring of STAGES slabs.
 #define LT_HGEMM2_TILES(X)    \
     X(256, 256, 32, 4, 2, 4)  \
-    X(256, 256, 32, 3, 2, 2)  \
-    X(256, 256, 32, 5, 2, 2)  \
+    X(256, 256, 32, 3, 2, 4)  \
+    X(256, 256, 32, 5, 2, 4)  \
     X(256, 128, 64, 3, 2, 2)  \
     X(128, 256, 64, 3, 2, 2)  \
     X(256, 128, 32, 4, 2, 2)  \
