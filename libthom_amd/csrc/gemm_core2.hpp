@@ -11,7 +11,11 @@
 // about 2.0 against 1.7 GHz under the board power limit), which is worth more
 // than its extra instructions.
 // Operand layout: lane l holds A[l % 16][8 (l / 16) .. +7] and
-// B[8 (l / 16) .. +7][l % 16]; it receives C[4 (l / 16) .. +3][l % 16].
+// B[8 (l / 16) .. +7][l % 16].  The instruction is given B's fragment as its
+// first operand and A's as its second, so it computes the transposed tile and
+// lane l receives C[l % 16][4 (l / 16) .. +3]: four consecutive columns of one
+// row, stored with one 8-byte write (the natural order would leave each lane
+// one column of four rows, four 2-byte writes).
 //
 //   A tile [BM][BK] fp16, K contiguous; 16-byte chunk kc of row r sits at
 //   kc ^ swzA(r).  An A fragment (8 consecutive k of one row) is one ds_read_b128.
@@ -267,10 +271,85 @@ constexpr int dma_at(int q) {
     return -1;
 }
 
+// four fp16 (rounded to nearest) to an 8-byte-aligned global address; the
+// global (not flat) store leaves the LDS counter alone
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) u32x2 global_u32x2_t;
+typedef __attribute__((address_space(1))) u32x4 global_u32x4_t;
+__device__ __forceinline__ void store_h4(__half* p, float a, float b, float c, float d) {
+    const unsigned lo = (unsigned)__half_as_ushort(__float2half_rn(a)) |
+                        ((unsigned)__half_as_ushort(__float2half_rn(b)) << 16);
+    const unsigned hi = (unsigned)__half_as_ushort(__float2half_rn(c)) |
+                        ((unsigned)__half_as_ushort(__float2half_rn(d)) << 16);
+    *(global_u32x2_t*)(p) = u32x2{lo, hi};
+}
+
+// Stores a wave's WM x WN tile (alpha * acc, fp16) at c (row stride ldc,
+// 16-byte aligned, ldc % 8 == 0) through its own slice of the idle ring:
+// 8-byte LDS writes in the MFMA layout, 16-byte reads of whole row segments,
+// 16-byte global stores (8 lanes cover 128 contiguous bytes of a row, so a
+// store instruction writes whole cache lines instead of 32-byte pieces of 16).
+// Rows go in passes of PR so that every wave's slice fits in the ring.
+// Chunk q of row r sits at q ^ f(r), f(r) = (r / (16 / CH)) % CH, which keeps
+// both the writes (32 lanes: 16 rows x 2 half-chunks) and the reads (16 lanes:
+// 256 contiguous bytes) free of bank conflicts.  LDS accesses of one wave
+// execute in order, so no barrier is needed between the passes.
 template <class C>
+__device__ __forceinline__ void store_tile_staged(char* smem, const v4f (&acc)[C::TM][C::TN], __half* c, long ldc,
+                                                  float alpha) {
+    constexpr int WM = C::WM, WN = C::WN, CH = WN / 8;   // 16-byte chunks per staged row
+    static_assert(CH == 8 || CH == 16, "staged rows of 128 or 256 bytes");
+    constexpr int LIM = C::SMEM / (C::WAVES * WN * 2);
+    constexpr int PR = WM <= LIM ? WM : (WM / 2 <= LIM ? WM / 2 : (WM / 4 <= LIM ? WM / 4 : 16));
+    static_assert(PR % 16 == 0 && WM % PR == 0 && PR * WN * 2 * C::WAVES <= C::SMEM, "staging passes");
+    constexpr int RPI = 64 / CH;   // rows per 16-byte read instruction
+    // the lane index through an opaque copy: a persistent caller's loop would
+    // otherwise hoist every lane-dependent address below out of the loop and
+    // keep them all live (spilled) through the main loop
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    char* st = smem + wave * (PR * WN * 2);
+    const int m = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int pass = 0; pass < WM / PR; pass++) {
+#pragma unroll
+        for (int ii = 0; ii < PR / 16; ii++)
+#pragma unroll
+            for (int j = 0; j < C::TN; j++) {
+                const int i = pass * (PR / 16) + ii;
+                const int r = 16 * ii + m;                       // row within the pass
+                const int q = 2 * j + (g >> 1), h = g & 1;       // chunk, 8-byte half
+                const int qs = q ^ ((r / (16 / CH)) % CH);
+                const unsigned lo = (unsigned)__half_as_ushort(__float2half_rn(alpha * acc[i][j][0])) |
+                                    ((unsigned)__half_as_ushort(__float2half_rn(alpha * acc[i][j][1])) << 16);
+                const unsigned hi = (unsigned)__half_as_ushort(__float2half_rn(alpha * acc[i][j][2])) |
+                                    ((unsigned)__half_as_ushort(__float2half_rn(alpha * acc[i][j][3])) << 16);
+                *reinterpret_cast<u32x2*>(st + r * (WN * 2) + qs * 16 + h * 8) = u32x2{lo, hi};
+            }
+#pragma unroll
+        for (int t = 0; t < PR / RPI; t++) {
+            const int r = RPI * t + lane / CH, q = lane % CH;
+            const int qs = q ^ ((r / (16 / CH)) % CH);
+            const u32x4 v = *reinterpret_cast<const u32x4*>(st + r * (WN * 2) + qs * 16);
+            *(global_u32x4_t*)(c + (long)(pass * PR + r) * ldc + q * 8) = v;
+        }
+    }
+}
+
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+// loop_done() runs once the products are in registers and the ring is idle,
+// before the stores (a persistent caller takes its next work item there, so
+// the ticket's latency hides behind the stores).
+template <class C, class Hook = NoHook>
 __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict__ A, long lda,
                                            const __half* __restrict__ B, long ldb, int M, int N, int K, int m0,
-                                           int n0, int s0, int s1, int split, int splits, const TileOut& out) {
+                                           int n0, int s0, int s1, int split, int splits, const TileOut& out,
+                                           const Hook& loop_done = Hook{}) {
     constexpr int TM = C::TM, TN = C::TN, BK = C::BK, STAGES = C::STAGES, KSTEPS = BK / 32;
     constexpr int MFMAS = TM * TN, NF = TM + TN;   // per k-step: MFMAs, fragment loads
     constexpr int D = C::LOADS / KSTEPS;            // DMA instructions per k-step
@@ -294,7 +373,10 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
     if (nslabs > 0) {
         Core2Src<C> src;
         src.init(A, lda, B, ldb, M, N, m0, n0, s0 * BK);
-        wait_vmcnt<0>();   // nothing older may be counted against the ring below
+        // Older memory operations still in flight (a persistent caller's stores
+        // of its previous tile) only make the ring's vmcnt waits stricter: a wait
+        // for at most n outstanding leaves at most n DMA instructions outstanding.
+        // So they drain while this tile's first slabs are fetched.
         // Ring of STAGES slabs.  During slab s, slab s+STAGES-1 is fetched into the
         // stage slab s-1 used (chunk kk of its DMA during k-step kk).  Every k-step
         // multiplies fragments loaded during the k-step before and loads the next
@@ -322,7 +404,7 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
             static_for<0, MFMAS>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 constexpr int i = q / TN, j = q % TN;
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fr[CUR].a[i], fr[CUR].b[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fr[CUR].b[j], fr[CUR].a[i], acc[i][j], 0, 0, 0);
                 constexpr int f = frag_at<NF, MFMAS>(q);
                 if constexpr (f >= 0) fr[CUR ^ 1].template load_one<f>(la, lb, lk, wm, wn);
                 constexpr int dd = dma_at<D, MFMAS>(q);
@@ -380,41 +462,79 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
             for (int i = 0; i < TM; i++)
 #pragma unroll
                 for (int j = 0; j < TN; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f.b[j], f.a[i], acc[i][j], 0, 0, 0);
         }
     }
+    loop_done();
     __syncthreads();   // the caller may refill the ring for its next tile
 
-    const int fcol = lane & 15, frow4 = 4 * (lane >> 4);
+    // lane l: row m0 + wm + 16 i + l % 16, columns n0 + wn + 16 j + 4 (l / 16) .. +3
+    const int frow = lane & 15, fcol4 = 4 * (lane >> 4);
     if (splits <= 1) {
+        // 8-byte stores need C 8-byte aligned and ldc % 4 == 0 (N % 8 == 0 keeps
+        // every group of four columns inside the matrix)
+        const bool vec = reinterpret_cast<uintptr_t>(out.C) % 8 == 0 && out.ldc % 4 == 0;
+        const bool whole = out.beta == 0.0f && m0 + C::BM <= M && n0 + C::BN <= N;
+        if (whole && reinterpret_cast<uintptr_t>(out.C) % 16 == 0 && out.ldc % 8 == 0) {
+            store_tile_staged<C>(smem, acc, out.C + (long)(m0 + wm) * out.ldc + n0 + wn, out.ldc, out.alpha);
+            return;
+        }
+        if (whole && vec) {   // no branches
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    const long row = m0 + wm + 16 * i + frow;
+                    const int col = n0 + wn + 16 * j + fcol4;
+                    store_h4(out.C + row * out.ldc + col, out.alpha * acc[i][j][0], out.alpha * acc[i][j][1],
+                             out.alpha * acc[i][j][2], out.alpha * acc[i][j][3]);
+                }
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < TM; i++)
 #pragma unroll
-            for (int j = 0; j < TN; j++)
+            for (int j = 0; j < TN; j++) {
+                const int row = m0 + wm + 16 * i + frow;
+                const int col = n0 + wn + 16 * j + fcol4;
+                if (row >= M || col >= N) continue;
+                __half* cp = out.C + (long)row * out.ldc + col;
+                float v[4];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int row = m0 + wm + 16 * i + frow4 + r;
-                    const int col = n0 + wn + 16 * j + fcol;
-                    if (row < M && col < N) {
-                        __half* cp = out.C + (long)row * out.ldc + col;
-                        float v = out.alpha * acc[i][j][r];
-                        if (out.beta != 0.0f) v = fmaf(out.beta, __half2float(*cp), v);
-                        *cp = __float2half_rn(v);
+                for (int r = 0; r < 4; r++) v[r] = out.alpha * acc[i][j][r];
+                if (vec && col + 3 < N) {
+                    if (out.beta != 0.0f) {
+                        const uint2 old = *reinterpret_cast<const uint2*>(cp);
+                        v[0] = fmaf(out.beta, __half2float(__ushort_as_half((unsigned short)(old.x & 0xFFFF))), v[0]);
+                        v[1] = fmaf(out.beta, __half2float(__ushort_as_half((unsigned short)(old.x >> 16))), v[1]);
+                        v[2] = fmaf(out.beta, __half2float(__ushort_as_half((unsigned short)(old.y & 0xFFFF))), v[2]);
+                        v[3] = fmaf(out.beta, __half2float(__ushort_as_half((unsigned short)(old.y >> 16))), v[3]);
                     }
+                    store_h4(cp, v[0], v[1], v[2], v[3]);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        if (col + r < N) {
+                            float x = v[r];
+                            if (out.beta != 0.0f) x = fmaf(out.beta, __half2float(cp[r]), x);
+                            cp[r] = __float2half_rn(x);
+                        }
                 }
+            }
         return;
     }
     float* mine = out.partial + (long)split * M * N;
 #pragma unroll
     for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < TN; j++)
+        for (int j = 0; j < TN; j++) {
+            const int row = m0 + wm + 16 * i + frow;
+            const int col = n0 + wn + 16 * j + fcol4;
+            if (row >= M) continue;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = m0 + wm + 16 * i + frow4 + r;
-                const int col = n0 + wn + 16 * j + fcol;
-                if (row < M && col < N) publish(mine + (long)row * N + col, acc[i][j][r]);
-            }
+            for (int r = 0; r < 4; r++)
+                if (col + r < N) publish(mine + (long)row * N + col + r, acc[i][j][r]);
+        }
     int* last_split = reinterpret_cast<int*>(smem);   // the ring is idle by now
     drain();
     __syncthreads();

@@ -39,7 +39,8 @@ struct GProb {
 };
 
 template <int BM, int BN, int BK>
-__global__ __launch_bounds__(256) void grouped_kernel(const GProb* __restrict__ probs, int count, int total,
+__global__ __launch_bounds__(256) void grouped_kernel(const GProb* __restrict__ probs,
+                                                      const int* __restrict__ item_prob, int count, int total,
                                                       unsigned* __restrict__ queue, float alpha, float beta) {
     __shared__ __align__(16) __half smem[2 * (BM + BN) * BK];
     __shared__ int item_s;
@@ -49,13 +50,8 @@ __global__ __launch_bounds__(256) void grouped_kernel(const GProb* __restrict__ 
         const int item = __builtin_amdgcn_readfirstlane(item_s);
         __syncthreads();
         if (item >= total) break;
-        // problem holding this item: last p with work_begin <= item
-        int lo = 0, hi = count - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (probs[mid].work_begin <= item) lo = mid; else hi = mid - 1;
-        }
-        const GProb& p = probs[lo];
+        (void)count;
+        const GProb& p = probs[item_prob[item]];
         const int local = item - p.work_begin;
         const int tile = local / p.splits, split = local % p.splits;
         // 8-row groups of tiles (L2 reuse of B panels)
@@ -88,27 +84,26 @@ __global__ __launch_bounds__(256) void grouped_kernel(const GProb* __restrict__ 
     }
 }
 
-// Same queue protocol with the DMA-ring tile (config.stages >= 3).  The work
-// item index travels through the first word of the ring, which is idle
-// between tiles.
+// Same queue protocol with the DMA-ring tile (config.stages >= 3), two
+// refinements: item_prob maps a work item straight to its problem (one load
+// instead of a dependent binary search), and a block takes its next ticket
+// when its products are in registers, so the atomic's latency hides behind
+// the stores.  The item index travels through the first word of the ring,
+// which is idle between tiles.
 template <class C>
-__global__ __launch_bounds__(C::THREADS) void grouped2_kernel(const GProb* __restrict__ probs, int count, int total,
+__global__ __launch_bounds__(C::THREADS) void grouped2_kernel(const GProb* __restrict__ probs,
+                                                               const int* __restrict__ item_prob, int count, int total,
                                                                unsigned* __restrict__ queue, float alpha, float beta) {
     __shared__ __align__(16) char smem[C::SMEM];
     int* item_s = reinterpret_cast<int*>(smem);
-    for (;;) {
-        if (threadIdx.x == 0) *item_s = (int)take_ticket(&queue[0]);
-        __syncthreads();
-        // uniform across the block: the table lookups below become scalar loads
-        const int item = __builtin_amdgcn_readfirstlane(*item_s);
-        __syncthreads();
-        if (item >= total) break;
-        int lo = 0, hi = count - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (probs[mid].work_begin <= item) lo = mid; else hi = mid - 1;
-        }
-        const GProb& p = probs[lo];
+    (void)count;
+    if (threadIdx.x == 0) *item_s = (int)take_ticket(&queue[0]);
+    __syncthreads();
+    // uniform across the block: the table lookups below become scalar loads
+    int item = __builtin_amdgcn_readfirstlane(*item_s);
+    __syncthreads();
+    while (item < total) {
+        const GProb& p = probs[item_prob[item]];
         const int local = item - p.work_begin;
         const int tile = local / p.splits, split = local % p.splits;
         constexpr int G = 8;
@@ -126,9 +121,18 @@ __global__ __launch_bounds__(C::THREADS) void grouped2_kernel(const GProb* __res
         o.beta = beta;
         o.partial = p.partial;
         o.counter = p.counters ? p.counters + tile : nullptr;
+        unsigned next = 0;
+        const auto take_next = [&]() {
+            if (threadIdx.x == 0) next = take_ticket(&queue[0]);
+        };
         gemm_tile2<C>(smem, p.A, (long)p.lda, p.B, (long)p.ldb, p.M, p.N, p.K, tm * C::BM, tn * C::BN, s0, s1, split,
-                      p.splits, o);
-        __syncthreads();
+                      p.splits, o, take_next);
+        // LDS-only barriers: the tile's stores drain during the next prologue
+        lds_barrier();   // everyone is done with the ring's first word
+        if (threadIdx.x == 0) *item_s = (int)next;
+        lds_barrier();
+        item = __builtin_amdgcn_readfirstlane(*item_s);
+        lds_barrier();   // read before the next prologue's DMA overwrites it
     }
     if (threadIdx.x == 0) {
         const unsigned done = take_ticket(&queue[1]);
@@ -139,7 +143,7 @@ __global__ __launch_bounds__(C::THREADS) void grouped2_kernel(const GProb* __res
     }
 }
 
-typedef void (*grouped_fn)(const GProb*, int, int, unsigned*, float, float);
+typedef void (*grouped_fn)(const GProb*, const int*, int, int, unsigned*, float, float);
 struct GTile2 {
     grouped_fn fn;
     int bm, bn, bk, stages, threads, smem;
@@ -179,6 +183,7 @@ struct lthip_grouped_plan {
     int grid;
     float alpha, beta;
     lt::GProb* table;       // device (inside the workspace)
+    int* item_prob;         // device: work item -> table row
     unsigned* queue;        // device, 2 counters
     hipGraph_t graph;
     hipGraphExec_t exec;
@@ -235,7 +240,8 @@ static int grouped_layout(const lthip_hgemm_problem* probs, int count, const lth
         work += (int)items;
     }
     (void)off_queue;
-    *ws_total = off_partial + partial_bytes;
+    // work item -> table row, after the partials
+    *ws_total = off_partial + align256(partial_bytes) + align256(sizeof(int) * (size_t)work);
     return LTHIP_OK;
 }
 
@@ -308,6 +314,18 @@ int lthip_hgemm_grouped_plan_create(const lthip_hgemm_problem* probs, int count,
     st = check_hip(hipMemcpyAsync(dtable, table.data(), sizeof(GProb) * table.size(), hipMemcpyHostToDevice, s),
                    "grouped: table upload");
     if (st != LTHIP_OK) return st;
+    std::vector<int> map((size_t)work);
+    for (size_t r = 0; r < table.size(); r++) {
+        const GProb& g = table[r];
+        const int end = g.work_begin + g.tiles_m * g.tiles_n * g.splits;
+        for (int it = g.work_begin; it < end; it++) map[(size_t)it] = (int)r;
+    }
+    int* dmap = reinterpret_cast<int*>(base + total - align256(sizeof(int) * (size_t)work));
+    if (work > 0) {
+        st = check_hip(hipMemcpyAsync(dmap, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice, s),
+                       "grouped: item map upload");
+        if (st != LTHIP_OK) return st;
+    }
     st = check_hip(hipStreamSynchronize(s), "grouped: table upload sync");
     if (st != LTHIP_OK) return st;
     lthip_grouped_plan* p = new (std::nothrow) lthip_grouped_plan;
@@ -322,6 +340,7 @@ int lthip_hgemm_grouped_plan_create(const lthip_hgemm_problem* probs, int count,
     p->alpha = __half2float(__ushort_as_half(alpha));
     p->beta = __half2float(__ushort_as_half(beta));
     p->table = dtable;
+    p->item_prob = dmap;
     p->queue = reinterpret_cast<unsigned*>(base);
     p->graph = nullptr;
     p->exec = nullptr;
@@ -333,7 +352,7 @@ int lthip_hgemm_grouped_run(lthip_grouped_plan* plan, void* stream) {
     using namespace lt;
     if (!plan) return fail(LTHIP_ERR_INVALID_ARG, "grouped: null plan");
     hipLaunchKernelGGL(plan->fn, dim3((unsigned)plan->grid), dim3((unsigned)plan->threads), 0, as_stream(stream),
-                       plan->table, plan->count, plan->total, plan->queue, plan->alpha, plan->beta);
+                       plan->table, plan->item_prob, plan->count, plan->total, plan->queue, plan->alpha, plan->beta);
     LT_TRY_LAUNCH("grouped hgemm launch");
     return LTHIP_OK;
 }
@@ -354,7 +373,7 @@ int lthip_hgemm_grouped_capture(lthip_grouped_plan* plan, void* stream) {
     int st = check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "grouped: begin capture");
     if (st != LTHIP_OK) return st;
     hipLaunchKernelGGL(plan->fn, dim3((unsigned)plan->grid), dim3((unsigned)plan->threads), 0, s, plan->table,
-                       plan->count, plan->total, plan->queue, plan->alpha, plan->beta);
+                       plan->item_prob, plan->count, plan->total, plan->queue, plan->alpha, plan->beta);
     hipError_t le = hipGetLastError();
     hipGraph_t g = nullptr;
     st = check_hip(hipStreamEndCapture(s, &g), "grouped: end capture");

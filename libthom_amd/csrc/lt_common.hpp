@@ -87,6 +87,10 @@ __device__ __forceinline__ float collect(const float* p) {
 }
 // all of this thread's memory operations have completed; a compiler barrier too
 __device__ __forceinline__ void drain() { asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory"); }
+// Workgroup barrier that orders LDS accesses only: global stores issued before
+// it may still be in flight afterwards (unlike __syncthreads(), whose release
+// fence waits for them).  For handing LDS words between threads.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 __device__ __forceinline__ unsigned take_ticket(unsigned* counter) {
     return __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
