@@ -293,13 +293,13 @@ __device__ __forceinline__ void store_h4(__half* p, float a, float b, float c, f
 // Rows go in passes of PR so that every wave's slice fits in the ring.
 // Chunk q of row r sits at q ^ f(r), f(r) = (r / (16 / CH)) % CH, which keeps
 // both the writes (32 lanes: 16 rows x 2 half-chunks) and the reads (16 lanes:
-// 256 contiguous bytes) free of bank conflicts.  LDS accesses of one wave
+// 256 contiguous bytes) free of bank conflicts for rows of 64, 128 or 256 bytes.  LDS accesses of one wave
 // execute in order, so no barrier is needed between the passes.
 template <class C>
 __device__ __forceinline__ void store_tile_staged(char* smem, const v4f (&acc)[C::TM][C::TN], __half* c, long ldc,
                                                   float alpha) {
     constexpr int WM = C::WM, WN = C::WN, CH = WN / 8;   // 16-byte chunks per staged row
-    static_assert(CH == 8 || CH == 16, "staged rows of 128 or 256 bytes");
+    static_assert(CH == 4 || CH == 8 || CH == 16, "staged rows of 64, 128 or 256 bytes");
     constexpr int LIM = C::SMEM / (C::WAVES * WN * 2);
     constexpr int PR = WM <= LIM ? WM : (WM / 2 <= LIM ? WM / 2 : (WM / 4 <= LIM ? WM / 4 : 16));
     static_assert(PR % 16 == 0 && WM % PR == 0 && PR * WN * 2 * C::WAVES <= C::SMEM, "staging passes");
@@ -405,8 +405,10 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
                 constexpr int q = decltype(qc)::value;
                 constexpr int i = q / TN, j = q % TN;
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fr[CUR].b[j], fr[CUR].a[i], acc[i][j], 0, 0, 0);
+                // loads in the order of first use by the next k-step: the B
+                // fragments and A fragment 0 feed its first TN MFMAs
                 constexpr int f = frag_at<NF, MFMAS>(q);
-                if constexpr (f >= 0) fr[CUR ^ 1].template load_one<f>(la, lb, lk, wm, wn);
+                if constexpr (f >= 0) fr[CUR ^ 1].template load_one<(f < TN ? TM + f : f - TN)>(la, lb, lk, wm, wn);
                 constexpr int dd = dma_at<D, MFMAS>(q);
                 if constexpr (dd >= 0) {
                     constexpr int d = KK * D + dd;
