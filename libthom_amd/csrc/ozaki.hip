@@ -259,9 +259,14 @@ __global__ __launch_bounds__(256) void ozaki_fp8_kernel(const uint8_t* __restric
 
 // ---- level GEMMs ------------------------------------------------------------
 constexpr int kLvBM = 256, kLvBN = 128, kLvBK = 128, kLvStages = 3;
+// eight waves, two per SIMD (one wave's fragment reads and DMA run beside the
+// other's MFMAs), on a 4 x 2 grid of 64 x 64 wave tiles: 64 accumulators and
+// two fragment sets of 64 registers fit the 256 registers of a wave
+constexpr int kLvWaves = 8, kLvThreads = 64 * kLvWaves;
 constexpr int kLvA = kLvBM * kLvBK, kLvB = kLvBN * kLvBK, kLvStage = kLvA + kLvB;
-constexpr int kLvAInstr = kLvA / 4096, kLvBInstr = kLvB / 4096, kLvLoads = kLvAInstr + kLvBInstr;
-constexpr int kLvTM = 8, kLvTN = 4;   // 16x16 MFMA tiles per wave: 128 x 64
+constexpr int kLvAInstr = kLvA / (1024 * kLvWaves), kLvBInstr = kLvB / (1024 * kLvWaves);
+constexpr int kLvLoads = kLvAInstr + kLvBInstr;
+constexpr int kLvTM = 4, kLvTN = 4;   // 16x16 MFMA tiles per wave: 64 x 64
 
 // 128-byte rows, 16-byte chunk c of row r at c ^ ((r >> 1) & 7): the 16 rows a
 // fragment read touches fall on 64 distinct banks
@@ -299,7 +304,7 @@ __device__ __forceinline__ void lv_mfma(v4f (&acc)[kLvTM][kLvTN], const LvFrag& 
 }
 
 // blockIdx.y = S-1-l: the longest levels start first.  lv[l][M][N] = level sums.
-__global__ __launch_bounds__(256) void ozaki_level_kernel(const uint8_t* __restrict__ Ad,
+__global__ __launch_bounds__(kLvThreads) void ozaki_level_kernel(const uint8_t* __restrict__ Ad,
                                                           const uint8_t* __restrict__ Bd, float* __restrict__ lv,
                                                           int M, int N, int K, int S, int tiles_m, int tiles_n) {
     __shared__ __align__(16) char smem[kLvStages * kLvStage];
@@ -309,7 +314,7 @@ __global__ __launch_bounds__(256) void ozaki_level_kernel(const uint8_t* __restr
     const int row0 = tm * kLvBM, col0 = tn * kLvBN;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wm = (wave >> 1) * 128, wn = (wave & 1) * 64;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const long sa = (long)M * K, sb = (long)N * K;
     const int kc = K / kLvBK;           // slabs per slice pair
     const int n = (l + 1) * kc;         // slabs of this level
@@ -556,7 +561,7 @@ int lthip_dgemm_ozaki(const double* A, const double* B, double* C, size_t M, siz
     if (oz_levels_fit(M, N, K)) {
         float* lv = reinterpret_cast<float*>(w + l.lv);
         const int tiles_m = m / kLvBM, tiles_n = n / kLvBN;
-        hipLaunchKernelGGL(ozaki_level_kernel, dim3((unsigned)(tiles_m * tiles_n), (unsigned)slices), dim3(256), 0, s,
+        hipLaunchKernelGGL(ozaki_level_kernel, dim3((unsigned)(tiles_m * tiles_n), (unsigned)slices), dim3(kLvThreads), 0, s,
                            ad, bd, lv, m, n, k, slices, tiles_m, tiles_n);
         LT_TRY_LAUNCH("ozaki level launch");
         const long total = (long)m * n;
