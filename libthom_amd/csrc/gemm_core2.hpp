@@ -72,11 +72,18 @@ __device__ __forceinline__ void wait_vmcnt() {
 // Ring hand-over: this wave's DMA for the slab about to be read has landed
 // (at most n newer DMA instructions outstanding), its own LDS reads are done,
 // and every wave got here.  Unlike __syncthreads() this does not wait for the
-// newer slabs still in flight.  The memory clobber keeps the compiler from
-// moving LDS accesses across it.
+// newer slabs still in flight.  The waits are an s_waitcnt builtin, so the
+// compiler's own wait insertion knows that no LDS read is outstanding after
+// it (after a wait written as inline asm it assumed every earlier read might
+// be, and waited for all of them, the next fragments' reads included, before
+// the first MFMA of every slab).  The empty asm before the wait and the
+// barrier's own asm carry memory clobbers, so no memory access moves across.
 template <int n>
 __device__ __forceinline__ void ring_barrier() {
-    asm volatile("s_waitcnt vmcnt(%0)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::"n"(n) : "memory");
+    static_assert(n >= 0 && n < 64, "vmcnt range");
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt((n & 0xF) | ((n >> 4) << 14) | (0x7 << 4));   // vmcnt(n) lgkmcnt(0)
+    asm volatile("s_barrier" ::: "memory");
 }
 
 template <int BM_, int BN_, int BK_, int STAGES_, int WGM_, int WGN_>

@@ -165,17 +165,20 @@ __device__ __forceinline__ v8h dq(const W4Frags<C>& f, int j, const uint32_t* zz
 }
 
 // scales and zero-points of this lane's columns for the slab in stage `ss`
+// (without zero-points, zmask 0 and zor 8 make every z 8: selected
+// arithmetically, since a branch here splits the k-loop into blocks whose
+// joins cost the compiler's wait insertion its knowledge of the LDS reads)
 template <class C>
-__device__ __forceinline__ void w4_read_sz(const char* ss, int cg, bool sym, uint32_t* zz, uint32_t* s2) {
+__device__ __forceinline__ void w4_read_sz(const char* ss, int cg, uint32_t zmask, uint32_t zor, uint32_t* zz,
+                                           uint32_t* s2) {
     const int lane = threadIdx.x & 63;
     const char* blk = ss + cg * C::SZ_STRIDE;
 #pragma unroll
     for (int j = 0; j < C::TN; j++) {
         const int c = 32 * j + (lane & 31);   // column within the wave's WN
         const uint32_t s = *reinterpret_cast<const uint16_t*>(blk + 2 * c);
-        uint32_t z = 8;
-        if (!sym) z = (*reinterpret_cast<const uint32_t*>(blk + C::SZ_SCALE + 4 * (c >> 3)) >> (4 * (c & 7))) & 0xFu;
-        zz[j] = zz_nat(z);
+        const uint32_t zw = *reinterpret_cast<const uint32_t*>(blk + C::SZ_SCALE + 4 * (c >> 3));
+        zz[j] = zz_nat(((zw >> (4 * (c & 7))) & zmask) | zor);
         s2[j] = s | (s << 16);
     }
 }
@@ -192,7 +195,7 @@ __device__ __forceinline__ void w4_tile2(char* smem, const __half* __restrict__ 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int cg = wave % C::WGN;
     const int wm = (wave / C::WGN) * C::WM, wn = cg * C::WN;
-    const bool sym = p.qzeros == nullptr;
+    const uint32_t zmask = p.qzeros ? 0xFu : 0u, zor = p.qzeros ? 0u : 8u;
 
     v16f acc[TM][TN];
 #pragma unroll
@@ -213,7 +216,7 @@ __device__ __forceinline__ void w4_tile2(char* smem, const __half* __restrict__ 
 
         uint32_t zz[TN], s2[TN];
         W4Frags<C> fr[2];
-        w4_read_sz<C>(smem + C::A_BYTES + C::B_BYTES, cg, sym, zz, s2);
+        w4_read_sz<C>(smem + C::A_BYTES + C::B_BYTES, cg, zmask, zor, zz, s2);
         fr[0].load(smem, smem + C::A_BYTES, 0, wm, wn);
         int stage = 0, fill_stage = STAGES - 1;
         for (int s = 0; s < nslabs; s++) {
@@ -256,7 +259,7 @@ __device__ __forceinline__ void w4_tile2(char* smem, const __half* __restrict__ 
                 stage = stage + 1 == STAGES ? 0 : stage + 1;
                 fill_stage = fill_stage + 1 == STAGES ? 0 : fill_stage + 1;
                 const char* na = smem + stage * C::STAGE_BYTES;
-                w4_read_sz<C>(na + C::A_BYTES + C::B_BYTES, cg, sym, zz, s2);
+                w4_read_sz<C>(na + C::A_BYTES + C::B_BYTES, cg, zmask, zor, zz, s2);
                 fr[0].load(na, na + C::A_BYTES, 0, wm, wn);   // KSTEPS is even
 #pragma unroll
                 for (int i = TM / 2; i < TM; i++)
