@@ -111,6 +111,133 @@ __global__ __launch_bounds__(kQThreads) void w4_quantize_kernel(const __half* __
     }
 }
 
+// Fast path (group 64, 128 or 256; 16-byte aligned W, packed words and
+// scales): one wave owns one group of G = 8R rows x 64 columns, entirely in
+// registers -- no LDS, no barrier.  Lane (rb, co) = (lane / 8, lane % 8) holds
+// rows rb R .. +R-1 of columns 8 co .. +7 (R 16-byte loads; each load
+// instruction covers 8 rows x 128 contiguous bytes).  Column extremes: packed
+// fp16 max/min over the lane's rows (exact, NaN ignored as fmaxf does), then
+// across the 8 row blocks with three xor shuffles.  Every lane of a column
+// octet derives the same 8 scales (and zero-points) with the IEEE divisions
+// of the host quantizer, and packs its own R/8 words per column.
+//
+// The per-weight division: w and s are fp16, so the exact quotient w / s is
+// either exactly a half-integer or at least ~2^-14 away from one (for
+// |w / s| <= 16, against an fp32 ulp of 2^-19 there).  Hence w * rcp(s), a few
+// ulp from w / s, rounds to the same integer as rintf(w / s) unless the
+// quotient is an exact tie, which h s == w detects exactly for the nearest
+// half-integer h (h s has at most 17 significant bits).  Beyond |w / s| = 16
+// the clamp to [0, 15] decides.  (Checked against IEEE division over every
+// finite fp16 w and 2583 fp16 scales: no difference.)
+typedef _Float16 qh2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ qh2_t as_qh2(uint32_t v) { return __builtin_bit_cast(qh2_t, v); }
+__device__ __forceinline__ uint32_t qh2_bits(qh2_t v) { return __builtin_bit_cast(uint32_t, v); }
+
+__device__ __forceinline__ int quant_round(float w, float s, float rs) {
+    const float q = w * rs;
+    const float h = floorf(q) + 0.5f;
+    return (int)rintf(h * s == w ? h : q);
+}
+
+template <bool ASYM, int R>
+__global__ __launch_bounds__(256) void w4_quantize_wave_kernel(const __half* __restrict__ W,
+                                                               uint32_t* __restrict__ packed,
+                                                               __half* __restrict__ scales,
+                                                               uint32_t* __restrict__ qzeros, int N, int groups) {
+    static_assert(R % 8 == 0 && R >= 8 && R <= 32, "rows per lane: whole packed words");
+    const int lane = threadIdx.x & 63;
+    const int tiles_n = (N + 63) / 64;
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int g = wv / tiles_n, tn = wv % tiles_n;
+    if (g >= groups) return;
+    const int rb = lane >> 3, co = lane & 7;
+    const int n = tn * 64 + 8 * co;
+    const bool colok = n < N;
+    const long row0 = (long)g * 8 * R + (long)rb * R;
+    uint4 v[R];
+#pragma unroll
+    for (int i = 0; i < R; i++)
+        v[i] = colok ? ld_nt(reinterpret_cast<const uint4*>(W + (row0 + i) * N + n)) : make_uint4(0, 0, 0, 0);
+    // per column pair p (columns 2p, 2p+1): hi = max(0, |w|) (symmetric) or
+    // max(0, w) and lo = min(0, w) (asymmetric); elementwise max/min follow
+    // maxNum/minNum (a NaN is ignored, as fmaxf/fminf do)
+    qh2_t hi[4], lo[4];
+#pragma unroll
+    for (int p = 0; p < 4; p++) hi[p] = lo[p] = qh2_t{(_Float16)0.0f, (_Float16)0.0f};
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        const uint32_t wd[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const qh2_t x = as_qh2(wd[p]);
+            if (ASYM) {
+                hi[p] = __builtin_elementwise_max(hi[p], x);
+                lo[p] = __builtin_elementwise_min(lo[p], x);
+            } else {
+                hi[p] = __builtin_elementwise_max(hi[p], __builtin_elementwise_abs(x));
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1)
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            hi[p] = __builtin_elementwise_max(hi[p], as_qh2((uint32_t)__shfl_xor((int)qh2_bits(hi[p]), off)));
+            if (ASYM)
+                lo[p] = __builtin_elementwise_min(lo[p], as_qh2((uint32_t)__shfl_xor((int)qh2_bits(lo[p]), off)));
+        }
+    float sc[8], rs[8];
+    int z[8];
+    uint32_t shalf[4] = {0, 0, 0, 0}, zword = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const float hj = (float)hi[j / 2][j & 1];
+        __half hs;
+        int zj = 8;
+        if (ASYM) {
+            const float lj = (float)lo[j / 2][j & 1];
+            const float r = hj - lj;
+            hs = __float2half(r > 0.0f ? __fdiv_rn(r, 15.0f) : 1.0f);
+            const int zz = (int)rintf(__fdiv_rn(-lj, __half2float(hs)));
+            zj = zz < 0 ? 0 : (zz > 15 ? 15 : zz);
+        } else {
+            const float sv = __fdiv_rn(2.0f * hj, 15.0f);   // quantize.cpp:23
+            hs = __float2half(sv > 0.0f ? sv : 1.0f);
+        }
+        sc[j] = __half2float(hs);
+        rs[j] = __builtin_amdgcn_rcpf(sc[j]);
+        z[j] = zj;
+        shalf[j / 2] |= (uint32_t)__half_as_ushort(hs) << (16 * (j & 1));
+        zword |= (uint32_t)zj << (4 * j);
+    }
+    if (colok && rb == 0) {
+        *reinterpret_cast<uint4*>(scales + (long)g * N + n) = make_uint4(shalf[0], shalf[1], shalf[2], shalf[3]);
+        if (ASYM) qzeros[(long)g * (N / 8) + n / 8] = zword;
+    }
+    if (!colok) return;
+#pragma unroll
+    for (int kw = 0; kw < R / 8; kw++) {
+        uint32_t word[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) word[j] = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint4 rv = v[8 * kw + i];
+            const uint32_t wd[4] = {rv.x, rv.y, rv.z, rv.w};
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const float w = (float)as_qh2(wd[j / 2])[j & 1];
+                int q = quant_round(w, sc[j], rs[j]) + z[j];
+                q = q < 0 ? 0 : (q > 15 ? 15 : q);
+                word[j] |= (uint32_t)q << (4 * i);
+            }
+        }
+        uint32_t* dst = packed + (row0 / 8 + kw) * (long)N + n;
+        reinterpret_cast<uint4*>(dst)[0] = make_uint4(word[0], word[1], word[2], word[3]);
+        reinterpret_cast<uint4*>(dst)[1] = make_uint4(word[4], word[5], word[6], word[7]);
+    }
+}
+
 }  // namespace
 }  // namespace lt
 
@@ -128,6 +255,26 @@ extern "C" int lthip_w4_quantize(const uint16_t* W, uint32_t* W_packed, uint16_t
     if (reinterpret_cast<uintptr_t>(W) % 16 || reinterpret_cast<uintptr_t>(W_packed) % 4 ||
         reinterpret_cast<uintptr_t>(scales) % 2 || reinterpret_cast<uintptr_t>(qzeros) % 4)
         return fail(LTHIP_ERR_INVALID_ARG, "w4_quantize: misaligned pointer");
+    // the register-resident fast path
+    const bool aligned = reinterpret_cast<uintptr_t>(W_packed) % 16 == 0 && reinterpret_cast<uintptr_t>(scales) % 16 == 0;
+    if (aligned && (group == 64 || group == 128 || group == 256)) {
+        const long waves = (long)(K / (size_t)group) * (long)((N + 63) / 64);
+        const dim3 grid((unsigned)((waves + 3) / 4));
+        const __half* w = reinterpret_cast<const __half*>(W);
+        __half* sp = reinterpret_cast<__half*>(scales);
+        const int groups = (int)(K / (size_t)group);
+#define LT_WQ(A, R_)                                                                                    \
+    hipLaunchKernelGGL((w4_quantize_wave_kernel<A, R_>), grid, dim3(256), 0, as_stream(stream), w, W_packed, sp, \
+                       qzeros, (int)N, groups)
+        if (mode == 1) {
+            if (group == 64) LT_WQ(true, 8); else if (group == 128) LT_WQ(true, 16); else LT_WQ(true, 32);
+        } else {
+            if (group == 64) LT_WQ(false, 8); else if (group == 128) LT_WQ(false, 16); else LT_WQ(false, 32);
+        }
+#undef LT_WQ
+        LT_TRY_LAUNCH("w4_quantize launch");
+        return LTHIP_OK;
+    }
     const size_t lds = quantize_lds(group);
     const void* fn = mode == 1 ? reinterpret_cast<const void*>(w4_quantize_kernel<true>)
                                : reinterpret_cast<const void*>(w4_quantize_kernel<false>);

@@ -54,6 +54,20 @@ def test_quantize_edges(dev, orc, asym):
     check(orc, W, group, asym, dev)
 
 
+@pytest.mark.parametrize("asym", [False, True])
+def test_quantize_ties_every_column(dev, orc, asym):
+    # group 128 (the register-resident kernel): every column holds +-7.5 f and
+    # the half-integers (k + 0.5) f, so max -> scale f exactly and W / scale
+    # lands on exact ties (round half to even) in every column, with scales
+    # that are not powers of two
+    K, N, group = 256, 136, 128
+    f = np.array([0.25, 1.0, 1.5, 3.0, 0.75, 1.25, 0.375, 2.5], np.float32)[np.arange(N) % 8]
+    k = np.arange(K) % 16 - 8 + 0.5                               # -7.5 .. 7.5
+    W = (k[:, None] * f[None, :]).astype(np.float16)
+    assert np.array_equal(W.astype(np.float32), (k[:, None] * f[None, :]).astype(np.float32))
+    check(orc, W, group, asym, dev)
+
+
 def test_quantize_large_group(dev, orc):
     # group 1024: the staged tile needs more than 64 KiB of LDS
     W = np.random.default_rng(41).standard_normal((2048, 64)).astype(np.float16)
