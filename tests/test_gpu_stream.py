@@ -64,6 +64,36 @@ def test_reduction_exact_integers_and_deterministic(dev):
     assert torch.equal(a, b)     # fixed order, and the ticket counter was reset
 
 
+def test_reduction_resets_counter(dev):
+    # the last block resets the ticket counter (the workspace's first 256
+    # bytes hold it): zero again after each call, small and large grids; a NaN
+    # or an infinity in the input reaches the result
+    from libthom_amd._lib import lib
+    L = lib()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    for n, bad in ((1_000_000, None), (60_000_000, None), (3, None), (1_000_000, float("nan")),
+                   (2_000_001, float("inf"))):
+        x = torch.rand(n, device=dev)
+        if bad is not None:
+            x[n // 2] = bad
+        need = L.lthip_reduce_sum_f32_workspace_size(n)
+        ws = torch.zeros(need, dtype=torch.uint8, device=dev)
+        out = torch.empty(1, dtype=torch.float32, device=dev)
+        print(f"reduction workspace n={n} bad={bad}", flush=True)
+        for _ in range(2):
+            assert L.lthip_reduce_sum_f32(x.data_ptr(), n, out.data_ptr(), ws.data_ptr(), need, stream) == 0
+            torch.cuda.synchronize()
+            assert int(ws[:256].count_nonzero()) == 0, n
+        got = out.item()
+        if bad is None:
+            ref = float(x.double().sum())
+            assert abs(got - ref) <= 1e-5 * ref, (n, got, ref)
+        elif bad != bad:
+            assert got != got
+        else:
+            assert got == bad
+
+
 @pytest.mark.parametrize("n", [1, 7, 4096, 1_000_000, 9_999_999])
 def test_sdot_vs_oracle(dev, orc, n):
     x = inputs.uniform_f32(n, -1.0, 1.0, 1234, inputs.S_SDOT_X)
