@@ -352,7 +352,7 @@ struct NoHook {
 // loop_done() runs once the products are in registers and the ring is idle,
 // before the stores (a persistent caller takes its next work item there, so
 // the ticket's latency hides behind the stores).
-template <class C, class Hook = NoHook>
+template <class C, bool SKIP = false, class Hook = NoHook>
 __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict__ A, long lda,
                                            const __half* __restrict__ B, long ldb, int M, int N, int K, int m0,
                                            int n0, int s0, int s1, int split, int splits, const TileOut& out,
@@ -364,7 +364,25 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
     static_assert(NF <= MFMAS && D <= MFMAS, "at most one fragment load and one DMA instruction per MFMA");
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wm = (wave / C::WGN) * C::WM, wn = (wave % C::WGN) * C::WN;
+    // SKIP (for many small problems, where edge tiles are common): waves w
+    // and w + 4 share a SIMD, and with eight waves (WGM even) the upper half
+    // takes the wave tiles of the lower half's partners in the other row half
+    // and, reversed, the other column half; so a tile that crosses the
+    // problem's last row or column in its first half leaves every SIMD one
+    // wave with work instead of two on some SIMDs and none on others.  A wave
+    // whose whole WM x WN sub-tile lies outside the problem then skips its
+    // fragment reads and MFMAs (it still issues its DMA share and takes every
+    // barrier).  Without SKIP the roles are in wave order and every wave
+    // multiplies (about 1 % faster on whole tiles).
+    int wr = wave / C::WGN, wc = wave % C::WGN;
+    if constexpr (SKIP && C::WAVES == 8 && C::WGM % 2 == 0) {
+        if (wave >= 4) {
+            wr = (wave - 4) / C::WGN + C::WGM / 2;
+            wc = C::WGN - 1 - (wave - 4) % C::WGN;
+        }
+    }
+    const int wm = wr * C::WM, wn = wc * C::WN;
+    const bool active = !SKIP || (m0 + wm < M && n0 + wn < N);
 
     v4f acc[TM][TN];
 #pragma unroll
@@ -397,58 +415,75 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
         ring_barrier<C::LOADS * (STAGES - 2)>();
 
         Core2Frags<C> fr[2];
-        fr[0].load(smem, smem + C::A_BYTES, 0, wm, wn);
+        if (active) fr[0].load(smem, smem + C::A_BYTES, 0, wm, wn);
         int stage = 0;                  // s % STAGES
         int fill_stage = STAGES - 1;    // (s + STAGES - 1) % STAGES
         // one k-step on fr[CUR], loading fr[CUR ^ 1] from (la, lb, lk) and
         // issuing DMA chunk KK of slab `fill`; the order of MFMAs, fragment loads
         // and DMA instructions is fixed by a scheduling fence after each MFMA
-        auto kstep = [&](auto curc, auto kkc, const char* la, const char* lb, int lk, int fill) {
+        // (ACT false: the DMA instructions and barriers only)
+        auto kstep = [&](auto actc, auto curc, auto kkc, const char* la, const char* lb, int lk, int fill) {
+            constexpr bool ACT = decltype(actc)::value != 0;
             constexpr int CUR = decltype(curc)::value, KK = decltype(kkc)::value;
             if constexpr (KK == KSTEPS - 1)   // slab s+1 landed, newest DMA: chunks 0..KK-1 of slab s+STAGES-1
                 ring_barrier<C::LOADS * (STAGES - 3) + KK * D>();
             __builtin_amdgcn_sched_barrier(0);
-            static_for<0, MFMAS>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                constexpr int i = q / TN, j = q % TN;
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fr[CUR].b[j], fr[CUR].a[i], acc[i][j], 0, 0, 0);
-                // loads in the order of first use by the next k-step: the B
-                // fragments and A fragment 0 feed its first TN MFMAs
-                constexpr int f = frag_at<NF, MFMAS>(q);
-                if constexpr (f >= 0) fr[CUR ^ 1].template load_one<(f < TN ? TM + f : f - TN)>(la, lb, lk, wm, wn);
-                constexpr int dd = dma_at<D, MFMAS>(q);
-                if constexpr (dd >= 0) {
-                    constexpr int d = KK * D + dd;
+            if constexpr (ACT) {
+                static_for<0, MFMAS>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    constexpr int i = q / TN, j = q % TN;
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fr[CUR].b[j], fr[CUR].a[i], acc[i][j], 0, 0, 0);
+                    // loads in the order of first use by the next k-step: the B
+                    // fragments and A fragment 0 feed its first TN MFMAs
+                    constexpr int f = frag_at<NF, MFMAS>(q);
+                    if constexpr (f >= 0)
+                        fr[CUR ^ 1].template load_one<(f < TN ? TM + f : f - TN)>(la, lb, lk, wm, wn);
+                    constexpr int dd = dma_at<D, MFMAS>(q);
+                    if constexpr (dd >= 0) {
+                        constexpr int d = KK * D + dd;
+                        core2_issue<d, d + 1>(src, smem, fill_stage, fill);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                });
+            } else {
+                static_for<0, D>([&](auto dc) {
+                    constexpr int d = KK * D + decltype(dc)::value;
                     core2_issue<d, d + 1>(src, smem, fill_stage, fill);
-                }
+                });
                 __builtin_amdgcn_sched_barrier(0);
-            });
+            }
         };
-        auto slab = [&](auto c0, int s) {
+        auto slab = [&](auto actc, auto c0, int s) {
             constexpr int C0 = decltype(c0)::value;   // fragment buffer of the slab's first k-step
             const char* sa = smem + stage * C::STAGE_BYTES;
             const int fill = s + STAGES - 1 < nslabs ? s + STAGES - 1 : nslabs - 1;
             const int nstage = stage + 1 == STAGES ? 0 : stage + 1;
             const char* na = smem + nstage * C::STAGE_BYTES;
             if constexpr (KSTEPS == 1) {
-                kstep(int_c<C0>{}, int_c<0>{}, na, na + C::A_BYTES, 0, fill);
+                kstep(actc, int_c<C0>{}, int_c<0>{}, na, na + C::A_BYTES, 0, fill);
             } else {
-                kstep(int_c<C0>{}, int_c<0>{}, sa, sa + C::A_BYTES, 1, fill);
-                kstep(int_c<C0 ^ 1>{}, int_c<1>{}, na, na + C::A_BYTES, 0, fill);
+                kstep(actc, int_c<C0>{}, int_c<0>{}, sa, sa + C::A_BYTES, 1, fill);
+                kstep(actc, int_c<C0 ^ 1>{}, int_c<1>{}, na, na + C::A_BYTES, 0, fill);
             }
             stage = nstage;
             fill_stage = fill_stage + 1 == STAGES ? 0 : fill_stage + 1;
         };
-        if constexpr (KSTEPS == 2) {
-            for (int s = 0; s < nslabs; s++) slab(int_c<0>{}, s);
-        } else {
-            int s = 0;
-            for (; s + 1 < nslabs; s += 2) {
-                slab(int_c<0>{}, s);
-                slab(int_c<1>{}, s + 1);
+        auto slabs = [&](auto actc) {
+            if constexpr (KSTEPS == 2) {
+                for (int s = 0; s < nslabs; s++) slab(actc, int_c<0>{}, s);
+            } else {
+                int s = 0;
+                for (; s + 1 < nslabs; s += 2) {
+                    slab(actc, int_c<0>{}, s);
+                    slab(actc, int_c<1>{}, s + 1);
+                }
+                if (s < nslabs) slab(actc, int_c<0>{}, s);
             }
-            if (s < nslabs) slab(int_c<0>{}, s);
-        }
+        };
+        if (active)
+            slabs(int_c<1>{});
+        else
+            slabs(int_c<0>{});
         wait_vmcnt<0>();   // the redundant tail DMA must land before the LDS is reused
     }
     if (partial) {
@@ -458,7 +493,7 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
         ring_barrier<0>();
         const int kl = 8 * (lane >> 4);   // first k of this lane's 8 in a k-step
 #pragma unroll
-        for (int kk = 0; kk < KSTEPS; kk++) {
+        for (int kk = 0; kk < KSTEPS && active; kk++) {
             Core2Frags<C> f;
             f.load(smem, smem + C::A_BYTES, kk, wm, wn);
             if (32 * kk + kl >= rem) {
@@ -480,6 +515,7 @@ __device__ __forceinline__ void gemm_tile2(char* smem, const __half* __restrict_
     // lane l: row m0 + wm + 16 i + l % 16, columns n0 + wn + 16 j + 4 (l / 16) .. +3
     const int frow = lane & 15, fcol4 = 4 * (lane >> 4);
     if (splits <= 1) {
+        if (!active) return;   // nothing of this wave's sub-tile is in the problem
         // 8-byte stores need C 8-byte aligned and ldc % 4 == 0 (N % 8 == 0 keeps
         // every group of four columns inside the matrix)
         const bool vec = reinterpret_cast<uintptr_t>(out.C) % 8 == 0 && out.ldc % 4 == 0;

@@ -125,8 +125,8 @@ __global__ __launch_bounds__(C::THREADS) void grouped2_kernel(const GProb* __res
         const auto take_next = [&]() {
             if (threadIdx.x == 0) next = take_ticket(&queue[0]);
         };
-        gemm_tile2<C>(smem, p.A, (long)p.lda, p.B, (long)p.ldb, p.M, p.N, p.K, tm * C::BM, tn * C::BN, s0, s1, split,
-                      p.splits, o, take_next);
+        gemm_tile2<C, true>(smem, p.A, (long)p.lda, p.B, (long)p.ldb, p.M, p.N, p.K, tm * C::BM, tn * C::BN, s0, s1,
+                            split, p.splits, o, take_next);
         // LDS-only barriers: the tile's stores drain during the next prologue
         lds_barrier();   // everyone is done with the ring's first word
         if (threadIdx.x == 0) *item_s = (int)next;

@@ -79,10 +79,13 @@ def test_w4a16_config_shapes(dev, orc, M):
     A, packed, s, qz = problem(M)
     C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), group).cpu().numpy()
     Wdq = orc.w4_dequant(packed, s, qz, group)
-    rows = M if M <= 128 else 64
-    row0 = 0 if M <= 128 else M - 64
-    ref = orc.w4a16_rows(A, Wdq, row0, rows)
-    w4_close(C[row0:row0 + rows], ref)
+    # M = 2048: every column against the oracle on row ranges spread over all
+    # eight 256-row tiles (two of them across a tile edge) and the last 32 rows
+    ranges = [(0, M)] if M <= 128 else [(0, 8), (250, 12), (517, 8), (1020, 8), (1279, 6), (1540, 8),
+                                        (1790, 4), (2016, 32)]
+    for row0, rows in ranges:
+        ref = orc.w4a16_rows(A, Wdq, row0, rows)
+        w4_close(C[row0:row0 + rows], ref)
 
 
 @pytest.mark.parametrize("cfg", [W4A16Config(1, 64, 16, 1), W4A16Config(1, 64, 16, 4), W4A16Config(1, 64, 16, 32),
