@@ -208,11 +208,15 @@ typedef struct lthip_w4a16_config {
                         4 = decode on the vector ALUs (M <= 4, group % (2048 / tile_n) == 0,
                         M * K <= 32768, K * N / 2 < 2^31; no workspace),
                         5 = skinny-M MFMA (M <= 64, N >= 8, group % 32 == 0, group <= 256,
-                        K * N / 2 < 2^31; no workspace) */
+                        K * N / 2 < 2^31; no workspace),
+                        6 = stream-K over the pipelined 128 x 128 tile (kind 3's conditions; one
+                        block per CU, the tile x K-slab iterations cut evenly between blocks;
+                        pieces of a shared tile meet in the workspace, summed in a fixed
+                        order: deterministic) */
     int tile_n;      /* kind 1: 64 columns per block (fixed); kinds 2, 3: BN; kind 4: 64 or 32;
-                        kind 5: 64 */
-    int tile_m;      /* kinds 2, 3: BM; kinds 4, 5: ignored (row tiles from M) */
-    int split_k;     /* K splits (>= 1); > 1 needs the workspace; kinds 4, 5: 1 */
+                        kind 5: 64; kind 6: 128 */
+    int tile_m;      /* kinds 2, 3: BM; kinds 4, 5: ignored (row tiles from M); kind 6: 128 */
+    int split_k;     /* K splits (>= 1); > 1 needs the workspace; kinds 4, 5, 6: 1 */
 } lthip_w4a16_config;
 
 size_t lthip_w4a16_workspace_size(size_t M, size_t N, size_t K, const lthip_w4a16_config *cfg);

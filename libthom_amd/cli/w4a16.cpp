@@ -386,9 +386,11 @@ static void tune(int argc, char* argv[]) {
         for (int s : {1, 2, 3, 4, 6, 8, 12, 16}) cands.push_back({1, 64, 16, s});
     for (auto bm_bn : {std::pair<int, int>{64, 64}, {64, 128}, {128, 128}, {128, 256}, {256, 128}, {256, 256}})
         cands.push_back({2, bm_bn.second, bm_bn.first, 1});
-    if (N % 32 == 0)
+    if (N % 32 == 0) {
         for (auto bm_bn : {std::pair<int, int>{128, 128}, {128, 256}, {256, 128}, {256, 256}})
             cands.push_back({3, bm_bn.second, bm_bn.first, 1});
+        cands.push_back({6, 128, 128, 1});
+    }
     float best = 1e30f;
     lthip_w4a16_config bestc{};
     for (const auto& c : cands) {

@@ -245,6 +245,11 @@ struct TileOut {
     float alpha, beta;   // C = alpha*acc + beta*C (beta == 0: C not read)
     float* partial;      // split-K: [splits][M][N] FP32 partials (or null)
     unsigned* counter;   // split-K: this tile's arrival counter
+    // Owner-collects split protocol (w4_tile2 only): when set, split s of the
+    // tile keeps its partial at fpart + s * fstride in the tile's MFMA
+    // fragment order, and the last split (the owner) waits for the others.
+    float* fpart = nullptr;
+    long fstride = 0;
 };
 
 // The last block of a split tile writes C[m0:m0+BM, n0:n0+BN] = alpha * (sum

@@ -134,6 +134,22 @@ __device__ __forceinline__ void interleave() {
     }
 }
 
+// interleave() with VA vector-ALU instructions spread between the MFMAs too
+// (the dequantisation of the next k-step in the look-ahead w4 tile).
+template <int MF, int DS, int VM, int VA, int Q = 0>
+__device__ __forceinline__ void interleave_va() {
+    if constexpr (Q < MF) {
+        constexpr int ds = (DS * (Q + 1)) / MF - (DS * Q) / MF;
+        constexpr int vm = (VM * (Q + 1)) / MF - (VM * Q) / MF;
+        constexpr int va = (VA * (Q + 1)) / MF - (VA * Q) / MF;
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if constexpr (ds > 0) __builtin_amdgcn_sched_group_barrier(0x100, ds, 0);
+        if constexpr (va > 0) __builtin_amdgcn_sched_group_barrier(0x002, va, 0);
+        if constexpr (vm > 0) __builtin_amdgcn_sched_group_barrier(0x020, vm, 0);
+        interleave_va<MF, DS, VM, VA, Q + 1>();
+    }
+}
+
 // Per-lane DMA source pointers for one tile; advanced by one slab per issue.
 template <class C>
 struct Core2Src {

@@ -183,9 +183,228 @@ __device__ __forceinline__ void w4_read_sz(const char* ss, int cg, uint32_t zmas
     }
 }
 
-// C tile at (m0, n0) over K-slabs [s0, s1) of 64 (K % 64 == 0); split-K
-// protocol as gemm_tile2 (alpha 1, beta 0).
+// Owner-collects hand-off of one split tile (out.fpart set).  Split s of the
+// tile writes its accumulators to plane s in fragment order: 16 floats per
+// lane and MFMA tile, 16-byte stores with the cache policy of an agent-scope
+// atomic store (sc1: written through past the XCD's L2; each instruction
+// writes 1 KiB in one piece), then drains them and
+// adds one to the tile's arrival counter (the ordering argument of publish /
+// drain / ticket).  The last split is the owner: it keeps its own sums in
+// registers, waits until the counter reads splits - 1, reads the other planes
+// (sc1 loads, issued only after that value came back and passed a barrier)
+// and adds them in split order, its own last: ((p0 + p1) + ...) + own, so the
+// result does not depend on arrival order.  It resets the counter for the next
+// call.  Returns true in the owner (acc then holds the tile's sum).
+// The wait is for pieces that run before the owner's own piece in their
+// blocks, in blocks dispatched before the owner's; it gives up after ~2^24
+// polls, marking the tile with NaN instead of hanging.
 template <class C>
+__device__ __forceinline__ bool w4_owner_collect(char* smem, v16f (&acc)[C::TM][C::TN], const TileOut& out,
+                                                 int split, int splits) {
+    constexpr int TM = C::TM, TN = C::TN, SC1 = 16;
+    constexpr int PLANE = C::BM * C::BN * 4;   // bytes
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // 16-byte piece q of fragment f (= wave TM TN + i TN + j) of lane l at
+    // ((4 f + q) 64 + l) 16 bytes: each store / load instruction covers 1 KiB
+    const int voff = (wave * TM * TN * 256 + lane) * 16;   // + ((i TN + j) 4 + q) * 1024
+    if (split < splits - 1) {
+        const __amdgpu_buffer_rsrc_t pr =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(out.fpart + (long)split * out.fstride), 0, PLANE, 0x00020000);
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const v4f v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
+                    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4i, v), pr,
+                                                           voff + ((i * TN + j) * 4 + q) * 1024, 0, SC1);
+                }
+        drain();
+        __syncthreads();
+        if (threadIdx.x == 0) take_ticket(out.counter);
+        return false;
+    }
+    // the flag lives in the idle ring: a second LDS object would cost the
+    // compiler's wait insertion its tracking of the ring's DMA in the k-loop
+    int* ok_flag = reinterpret_cast<int*>(smem);
+    if (threadIdx.x == 0) {
+        int ok = 0;
+        for (int poll = 0; poll < (1 << 24); poll++) {
+            if (__hip_atomic_load(out.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (unsigned)(splits - 1)) {
+                ok = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (ok) reset_counter(out.counter);
+        *ok_flag = ok;
+    }
+    __syncthreads();
+    if (!*ok_flag) {
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[i][j][r] = __builtin_nanf("");
+        return true;
+    }
+    v16f sum[TM][TN];
+    for (int sp = 0; sp < splits - 1; sp++) {
+        const __amdgpu_buffer_rsrc_t pr =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(out.fpart + (long)sp * out.fstride), 0, PLANE, 0x00020000);
+        v4f v[TM][TN][4];
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    v[i][j][q] = __builtin_bit_cast(
+                        v4f, __builtin_amdgcn_raw_buffer_load_b128(pr, voff + ((i * TN + j) * 4 + q) * 1024, 0, SC1));
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                        sum[i][j][4 * q + e] = sp == 0 ? v[i][j][q][e] : sum[i][j][4 * q + e] + v[i][j][q][e];
+    }
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[i][j][r] = sum[i][j][r] + acc[i][j][r];
+    return true;
+}
+
+// Main loop of the look-ahead tile (w4_tile2 with LA): nslabs >= 1 slabs
+// from slab s0, accumulated into acc.
+template <class C>
+__device__ __forceinline__ void w4_mainloop_la(char* smem, const __half* __restrict__ A, long lda, const BW4Params& p,
+                                               int M, int N, int m0, int n0, int s0, int nslabs, uint32_t zmask,
+                                               uint32_t zor, v16f (&acc)[C::TM][C::TN]) {
+    constexpr int TM = C::TM, TN = C::TN, BK = C::BK, STAGES = C::STAGES, KSTEPS = BK / 16;
+    constexpr int MFMAS = TM * TN;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int cg = wave % C::WGN;
+    const int wm = (wave / C::WGN) * C::WM, wn = cg * C::WN;
+    W4Src<C> src;
+    src.init(A, lda, p, M, N, m0, n0, s0 * BK);
+    wait_vmcnt<0>();
+#pragma unroll
+    for (int q = 0; q < STAGES - 1; q++) src.template issue<0, C::LOADS>(smem, q, q < nslabs ? q : nslabs - 1);
+    ring_barrier<C::LOADS * (STAGES - 2)>();
+    const int frow = lane & 31, fkc = lane >> 5;
+    // A fragments of k-step kk (two buffers) and weight words (two
+    // buffers), each read straight from the stage as W4Frags::load does
+    auto load_a = [&](v8h* a, const char* sa, int kk) {
+#pragma unroll
+        for (int i = 0; i < TM; i++) {
+            const int row = wm + 32 * i + frow;
+            const int kc = (2 * kk + fkc) ^ swzA2<C::BK>(row);
+            const uint4 raw = *reinterpret_cast<const uint4*>(sa + row * (C::BK * 2) + kc * 16);
+            __builtin_memcpy(&a[i], &raw, 16);
+        }
+    };
+    auto load_w = [&](uint32_t* w, const char* sb, int kk) {
+        const int r = 2 * kk + fkc;
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+            const int c = wn + 32 * j + frow;
+            w[j] = *reinterpret_cast<const uint32_t*>(sb + r * (C::BN * 4) + (((c >> 2) ^ swzW2(r)) << 4) +
+                                                      (c & 3) * 4);
+        }
+    };
+    auto dq_all = [&](v8h* b, const uint32_t* w, const uint32_t* zz, const uint32_t* s2) {
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+            const uint4 d = dequant8_nat(w[j], zz[j], s2[j]);
+            __builtin_memcpy(&b[j], &d, 16);
+        }
+    };
+    uint32_t zz[TN], s2[TN];
+    v8h a[2][TM], bq[2][TN];
+    uint32_t wr[2][TN];
+    w4_read_sz<C>(smem + C::A_BYTES + C::B_BYTES, cg, zmask, zor, zz, s2);
+    load_a(a[0], smem, 0);
+    load_w(wr[0], smem + C::A_BYTES, 0);
+    load_w(wr[1], smem + C::A_BYTES, 1);
+    dq_all(bq[0], wr[0], zz, s2);
+    int stage = 0, fill_stage = STAGES - 1;
+    for (int s = 0; s < nslabs; s++) {
+        const char* sa = smem + stage * C::STAGE_BYTES;
+        const char* sb = sa + C::A_BYTES;
+        const int fill = s + STAGES - 1 < nslabs ? s + STAGES - 1 : nslabs - 1;
+        __builtin_amdgcn_sched_barrier(0);
+        // k-steps 0 .. KSTEPS-2: read A(kk+1) and W(kk+2), dequantise
+        // W(kk+1), MFMAs of kk
+        static_for<0, KSTEPS - 1>([&](auto kkl) {
+            constexpr int kk = decltype(kkl)::value;
+            load_a(a[(kk + 1) & 1], sa, kk + 1);
+            if constexpr (kk + 2 < KSTEPS) load_w(wr[kk & 1], sb, kk + 2);
+            src.template issue<kk * C::LOADS / KSTEPS, (kk + 1) * C::LOADS / KSTEPS>(smem, fill_stage, fill);
+            dq_all(bq[(kk + 1) & 1], wr[(kk + 1) & 1], zz, s2);
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kk & 1][i], bq[kk & 1][j], acc[i][j], 0,
+                                                                       0, 0);
+            interleave_va<MFMAS, TM + (kk + 2 < KSTEPS ? TN : 0),
+                          (kk + 1) * C::LOADS / KSTEPS - kk * C::LOADS / KSTEPS, 16 * TN>();
+            __builtin_amdgcn_sched_barrier(0);
+        });
+        {
+            constexpr int KL = KSTEPS - 1;
+            src.template issue<KL * C::LOADS / KSTEPS, C::LOADS>(smem, fill_stage, fill);
+#pragma unroll
+            for (int i = 0; i < TM / 2; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[KL & 1][i], bq[KL & 1][j], acc[i][j], 0,
+                                                                       0, 0);
+            interleave<MFMAS / 2, 0, C::LOADS - KL * C::LOADS / KSTEPS>();
+            __builtin_amdgcn_sched_barrier(0);
+            ring_barrier<C::LOADS * (STAGES - 2)>();
+            stage = stage + 1 == STAGES ? 0 : stage + 1;
+            fill_stage = fill_stage + 1 == STAGES ? 0 : fill_stage + 1;
+            const char* na = smem + stage * C::STAGE_BYTES;
+            w4_read_sz<C>(na + C::A_BYTES + C::B_BYTES, cg, zmask, zor, zz, s2);
+            load_a(a[0], na, 0);   // KSTEPS is even: k-step 0 uses buffer 0
+            load_w(wr[0], na + C::A_BYTES, 0);
+            load_w(wr[1], na + C::A_BYTES, 1);
+#pragma unroll
+            for (int i = TM / 2; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[KL & 1][i], bq[KL & 1][j], acc[i][j], 0,
+                                                                       0, 0);
+            interleave<MFMAS / 2, TM + 2 * TN, 0>();
+            __builtin_amdgcn_sched_barrier(0);
+            dq_all(bq[0], wr[0], zz, s2);
+        }
+    }
+    wait_vmcnt<0>();
+}
+
+// C tile at (m0, n0) over K-slabs [s0, s1) of 64 (K % 64 == 0); split-K
+// protocol as gemm_tile2 (alpha 1, beta 0), or the owner-collects hand-off
+// above when out.fpart is set.
+//
+// LA (look-ahead): the weight words are read two k-steps ahead and
+// dequantised one k-step ahead, between the MFMAs of the current k-step, so
+// the vector work of a wave runs under its own MFMAs (with one wave per SIMD
+// nothing else fills the MFMA pipe while it dequantises).  Only the first
+// k-step of each slab, whose words and scales arrive after the ring barrier,
+// is dequantised after its reads.
+template <class C, bool LA = false>
 __device__ __forceinline__ void w4_tile2(char* smem, const __half* __restrict__ A, long lda, const BW4Params& p, int M,
                                          int N, int m0, int n0, int s0, int s1, int split, int splits,
                                          const TileOut& out) {
@@ -206,7 +425,9 @@ __device__ __forceinline__ void w4_tile2(char* smem, const __half* __restrict__ 
             for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
 
     const int nslabs = s1 - s0;
-    if (nslabs > 0) {
+    if constexpr (LA) {
+        if (nslabs > 0) w4_mainloop_la<C>(smem, A, lda, p, M, N, m0, n0, s0, nslabs, zmask, zor, acc);
+    } else if (nslabs > 0) {
         W4Src<C> src;
         src.init(A, lda, p, M, N, m0, n0, s0 * BK);
         wait_vmcnt<0>();
@@ -275,6 +496,37 @@ __device__ __forceinline__ void w4_tile2(char* smem, const __half* __restrict__ 
     __syncthreads();
 
     const int fcol = lane & 31, frow4 = 4 * (lane >> 5);
+    if (splits > 1 && out.fpart) {
+        if (!w4_owner_collect<C>(smem, acc, out, split, splits)) return;
+        splits = 1;   // the owner now holds the tile's sum: stored below
+    }
+    // Look-ahead tiles store C through the idle ring: the fp16 tile is
+    // assembled in LDS (rows padded by 8 halves against bank conflicts), then
+    // written row by row in 16-byte pieces.  The others store element-wise.
+    constexpr int LDT = C::BN + 8;
+    constexpr bool STAGED = LA && C::BM * LDT * 2 <= C::SMEM;
+    if (STAGED && splits <= 1 && reinterpret_cast<uintptr_t>(out.C) % 16 == 0 && out.ldc % 8 == 0) {
+        __half* ct = reinterpret_cast<__half*>(smem);
+        __syncthreads();   // every wave is past its last use of the ring
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+#pragma unroll
+                for (int r = 0; r < 16; r++)
+                    ct[(wm + 32 * i + 8 * (r >> 2) + frow4 + (r & 3)) * LDT + wn + 32 * j + fcol] =
+                        __float2half_rn(acc[i][j][r]);
+        __syncthreads();
+        constexpr int CH = C::BN / 8;
+        for (int e = threadIdx.x; e < C::BM * CH; e += C::THREADS) {
+            const int row = e / CH, ch = e % CH;
+            const int grow = m0 + row, gcol = n0 + 8 * ch;
+            if (grow < M && gcol < N)
+                *reinterpret_cast<uint4*>(out.C + (long)grow * out.ldc + gcol) =
+                    *reinterpret_cast<const uint4*>(ct + row * LDT + 8 * ch);
+        }
+        return;
+    }
     if (splits <= 1) {
 #pragma unroll
         for (int i = 0; i < TM; i++)

@@ -757,6 +757,115 @@ const Tiled2Entry* find_tiled2(int bm, int bn) {
     return nullptr;
 }
 
+// ---- kind 6: stream-K over the pipelined tile ---------------------------
+// The launch's (tile, K-slab) iterations, tiles x slabs of them in tile-major
+// order, are cut into gridDim.x contiguous ranges of equal length (+-1), one
+// per block, so every block does the same MFMA and dequantisation work
+// whatever the tile count.  A range covers the end of one tile, whole tiles
+// and the start of another; each piece is one w4_tile2 call whose split index
+// is the block's place among the blocks that share the tile.  The block that
+// holds a shared tile's last slab owns it: the other pieces publish FP32
+// partials in fragment order and the owner adds them to its own in split
+// order (w4_owner_collect; deterministic); a tile covered by one block is
+// stored directly.  Workspace: the counter head, then (most blocks sharing a
+// tile - 1) planes of tiles x BM x BN floats.
+// Block b covers iterations [floor(b T / G), floor((b + 1) T / G)), so the
+// block holding iteration i is floor(((i + 1) G - 1) / T).
+__device__ __forceinline__ int streamk_owner(long i, long T, int G) { return (int)(((i + 1) * G - 1) / T); }
+
+template <class C>
+__global__ __launch_bounds__(C::THREADS) void w4a16_streamk_kernel(const __half* __restrict__ A, BW4Params p,
+                                                                    __half* __restrict__ Cm, int M, int N, int K,
+                                                                    int tiles_m, int tiles, float* __restrict__ partial,
+                                                                    unsigned* __restrict__ counters) {
+    __shared__ __align__(16) char smem[C::SMEM];
+    const int S = K / C::BK;
+    const long T = (long)tiles * S;
+    const int G = gridDim.x, b = blockIdx.x;
+    auto piece = [&](long i0, long i1) {   // iterations [i0, i1) of one tile
+        const int t = (int)(i0 / S);
+        const long tbeg = (long)t * S;
+        const int first = streamk_owner(tbeg, T, G), last = streamk_owner(tbeg + S - 1, T, G);
+        TileOut o;
+        o.C = Cm;
+        o.ldc = N;
+        o.alpha = 1.0f;
+        o.beta = 0.0f;
+        o.partial = nullptr;
+        o.counter = counters + t;
+        o.fpart = partial + (long)t * (C::BM * C::BN);
+        o.fstride = (long)tiles * (C::BM * C::BN);
+        const int tm = t % tiles_m, tn = t / tiles_m;
+        w4_tile2<C, true>(smem, A, (long)K, p, M, N, tm * C::BM, tn * C::BN, (int)(i0 - tbeg), (int)(i1 - tbeg), b - first,
+                    last - first + 1, o);
+        // the next piece's DMA must not overwrite LDS words (the last-split
+        // flag) that a slower wave of this piece still reads
+        __syncthreads();
+    };
+    long i = (long)b * T / G, iend = (long)(b + 1) * T / G;
+    // The range's last piece, when it starts a tile without finishing it, goes
+    // first: its partial is then published early, and the piece that finishes
+    // a tile (the range's first) comes last, so that block usually arrives
+    // last at its tile and collects it without a second collect behind it.
+    const long lbeg = iend > i ? ((iend - 1) / S) * S : iend;
+    if (lbeg > i && iend < lbeg + S) {
+        piece(lbeg, iend);
+        iend = lbeg;
+    }
+    while (i < iend) {
+        const long tend = (i / S + 1) * S;
+        const long pend = iend < tend ? iend : tend;
+        piece(i, pend);
+        i = pend;
+    }
+}
+
+typedef void (*streamk_fn)(const __half*, BW4Params, __half*, int, int, int, int, int, float*, unsigned*);
+struct StreamKEntry {
+    int bm, bn, smem;
+    streamk_fn fn;
+    int threads;
+};
+#define LT_W4SK(bm, bn, st, wm, wn)                                                                     \
+    {bm, bn, Core2W4<bm, bn, st, wm, wn>::SMEM, w4a16_streamk_kernel<Core2W4<bm, bn, st, wm, wn>>, \
+     Core2W4<bm, bn, st, wm, wn>::THREADS},
+// four waves of 128 x 32 (all the tile's rows: a dequantised weight fragment
+// feeds four MFMAs), one per SIMD, 96 KiB of ring: one block per CU
+const StreamKEntry kStreamK[] = {LT_W4SK(128, 128, 4, 1, 4)};
+#undef LT_W4SK
+constexpr int kNumStreamK = sizeof(kStreamK) / sizeof(kStreamK[0]);
+
+const StreamKEntry* find_streamk(int bm, int bn) {
+    for (int i = 0; i < kNumStreamK; i++)
+        if (kStreamK[i].bm == bm && kStreamK[i].bn == bn) return &kStreamK[i];
+    return nullptr;
+}
+
+// Kind-6 launch geometry: G blocks (as many as fit on the CUs at once, at
+// most one per 4 iterations) and the most blocks sharing one tile (the
+// partial planes the workspace holds).
+struct StreamKPlan {
+    int tiles_m, tiles, grid, max_splits;
+};
+StreamKPlan streamk_plan(const StreamKEntry& e, long M, long N, long K) {
+    StreamKPlan pl;
+    pl.tiles_m = (int)((M + e.bm - 1) / e.bm);
+    pl.tiles = pl.tiles_m * (int)((N + e.bn - 1) / e.bn);
+    const long S = K / 64, T = (long)pl.tiles * S;
+    long per_cu = 163840 / (e.smem + 64);
+    if (per_cu < 1) per_cu = 1;
+    long G = (long)cu_count() * per_cu;
+    if (G > T / 4) G = T / 4;
+    if (G < 1) G = 1;
+    pl.grid = (int)G;
+    pl.max_splits = 1;
+    for (long t = 0; t < pl.tiles; t++) {
+        const long first = ((t * S + 1) * G - 1) / T, last = (((t + 1) * S) * G - 1) / T;
+        if (last - first + 1 > pl.max_splits) pl.max_splits = (int)(last - first + 1);
+    }
+    return pl;
+}
+
 // Tail split of a kind-3 launch with split_k == 1: `tail` splits for the
 // tiles after the last full round of launch slots (1: none).
 int tiled2_tail(const Tiled2Entry& e, long M, long N, long K, int* full) {
@@ -792,6 +901,16 @@ void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_
             c.kind = 5;
             c.tile_n = 64;
             c.tile_m = 0;
+            c.split_k = 1;
+        } else if (N % 32 == 0 && M <= 256) {
+            // stream-K over 128 x 128 tiles whose waves hold all rows (each
+            // weight dequantised once, every CU busy): at N = 11008, K = 4096,
+            // M = 128 in 27 against 32 us for the 64 x 128 pipelined tile,
+            // M = 256 in 40 against 46 us; the launch falls back to that tile
+            // where kind 6 does not take the shape
+            c.kind = 6;
+            c.tile_m = 128;
+            c.tile_n = 128;
             c.split_k = 1;
         } else if (N % 32 == 0) {
             // pipelined tiles: 64 x 128 up to M = 256 (at N = 11008, K = 4096:
@@ -832,6 +951,13 @@ size_t lthip_w4a16_workspace_size(size_t M, size_t N, size_t K, const lthip_w4a1
     lthip_w4a16_config c;
     lt::resolve(M, N, K, cfg, &c);
     int splits = c.split_k;
+    if (c.kind == 6) {
+        const lt::StreamKEntry* e = lt::find_streamk(c.tile_m, c.tile_n);
+        if (!e || K == 0) return 0;
+        const lt::StreamKPlan pl = lt::streamk_plan(*e, (long)M, (long)N, (long)K);
+        if (pl.max_splits <= 1) return 0;
+        return lt::kCounterBytes + sizeof(float) * (size_t)(pl.max_splits - 1) * pl.tiles * e->bm * e->bn;
+    }
     if (c.kind == 3 && splits <= 1) {
         const lt::Tiled2Entry* e = lt::find_tiled2(c.tile_m, c.tile_n);
         int full = 0;
@@ -859,7 +985,7 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
     BW4Params p{W_packed, reinterpret_cast<const __half*>(scales), qzeros, group};
     const int m = (int)M, n = (int)N, k = (int)K;
     hipStream_t s = as_stream(stream);
-    if ((c.kind == 4 || c.kind == 5) && K == 0) c.kind = 1;   // nothing to accumulate: the stream kernel writes zeros
+    if ((c.kind == 4 || c.kind == 5 || c.kind == 6) && K == 0) c.kind = 1;   // nothing to accumulate: the stream kernel writes zeros
     if (c.kind == 4) {
         const bool explicit4 = cfg && cfg->kind == 4;
         const DecodeEntry* e = decode_entry((long)M, (long)N, (long)K, group, c.tile_n);
@@ -954,6 +1080,45 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
 #undef LT_W4S
         LT_TRY_LAUNCH("w4a16 stream launch");
         return LTHIP_OK;
+    }
+    if (c.kind == 6) {
+        const bool explicit6 = cfg && cfg->kind == 6;
+        const StreamKEntry* e = find_streamk(c.tile_m, c.tile_n);
+        const bool ok = e && group % 64 == 0 && N % 32 == 0 && K > 0 &&
+                        reinterpret_cast<uintptr_t>(scales) % 16 == 0 &&
+                        reinterpret_cast<uintptr_t>(qzeros) % 16 == 0 &&
+                        ((M + e->bm - 1) / e->bm) * ((N + e->bn - 1) / e->bn) <= kMaxCounters;
+        if (!ok && explicit6)
+            return fail(LTHIP_ERR_UNSUPPORTED,
+                        "w4a16: kind 6 needs a built tile (128x128), K > 0, group %% 64 == 0, N %% 32 == 0, "
+                        "16-byte aligned scales/zero-points and at most %zu tiles",
+                        kMaxCounters);
+        if (explicit6 && c.split_k != 1) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: kind 6 takes split_k = 1");
+        if (ok) {
+            const StreamKPlan pl = streamk_plan(*e, (long)M, (long)N, (long)K);
+            float* partial = nullptr;
+            unsigned* counters = nullptr;
+            if (pl.max_splits > 1) {
+                const size_t need =
+                    kCounterBytes + sizeof(float) * (size_t)(pl.max_splits - 1) * pl.tiles * e->bm * e->bn;
+                if (!ws || ws_bytes < need) return fail(LTHIP_ERR_WORKSPACE, "w4a16: need %zu workspace bytes", need);
+                if (reinterpret_cast<uintptr_t>(ws) % 16)
+                    return fail(LTHIP_ERR_INVALID_ARG, "w4a16: workspace alignment");
+                counters = reinterpret_cast<unsigned*>(ws);
+                partial = reinterpret_cast<float*>(reinterpret_cast<char*>(ws) + kCounterBytes);
+            }
+            hipLaunchKernelGGL(e->fn, dim3((unsigned)pl.grid), dim3((unsigned)e->threads), 0, s,
+                               reinterpret_cast<const __half*>(A), p, reinterpret_cast<__half*>(C), m, n, k,
+                               pl.tiles_m, pl.tiles, partial, counters);
+            LT_TRY_LAUNCH("w4a16 stream-K launch");
+            return LTHIP_OK;
+        }
+        // automatic choice on a shape kind 6 does not take: the 64 x 128
+        // pipelined tile (which falls back to kind 2 in turn)
+        c.kind = 3;
+        c.tile_m = 64;
+        c.tile_n = 128;
+        c.split_k = 1;
     }
     if (c.kind == 3) {
         int idx2 = -1;

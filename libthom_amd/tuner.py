@@ -56,15 +56,16 @@ W4A16_SPACE: List[Tuple[int, int, int, int]] = (
     [(2, bn, bm, 1) for bm, bn in ((128, 256), (256, 128), (256, 256))] +
     [(3, 128, 128, 1), (3, 128, 128, 2), (3, 256, 128, 1), (3, 128, 256, 1), (3, 256, 256, 1)] +
     [(4, 64, 0, 1), (4, 32, 0, 1), (5, 64, 0, 1)] +
-    [(3, 128, 64, 1), (3, 128, 64, 2), (3, 64, 64, 1), (5, 32, 0, 1)])
+    [(3, 128, 64, 1), (3, 128, 64, 2), (3, 64, 64, 1), (5, 32, 0, 1)] +
+    [(6, 128, 128, 1)])
 
 
 def valid(cfg: Tuple[int, int, int, int], M: int, K: int, N: Optional[int] = None,
           group: Optional[int] = None) -> bool:
     """Whether the library runs cfg on the shape; kind 3 needs N % 32 == 0 and
     group % 64 == 0, kind 4 M <= 4, group % (2048 / tile_n) == 0 and M K <=
-    32768, kind 5 M <= 64 and group % 32 == 0 (kinds 3-5: false when N or
-    group is not given)."""
+    32768, kind 5 M <= 64 and group % 32 == 0, kind 6 kind 3's conditions
+    and at most 16384 tiles (kinds 3-6: false when N or group is not given)."""
     kind, tile_n, _, split = cfg
     if kind == 1:
         return M <= 128 and split <= max(1, K // 32)
@@ -77,6 +78,9 @@ def valid(cfg: Tuple[int, int, int, int], M: int, K: int, N: Optional[int] = Non
     if kind == 3:
         return N is not None and group is not None and N % 32 == 0 and group % 64 == 0 and \
             split <= max(1, K // 256)
+    if kind == 6:
+        return N is not None and group is not None and N % 32 == 0 and group % 64 == 0 and split == 1 and \
+            ((M + 127) // 128) * ((N + 127) // 128) <= 16384
     return split <= max(1, K // 256)
 
 

@@ -303,3 +303,69 @@ def test_w4a16_skinny_rejects_unsupported(dev):
     with pytest.raises(LthipError):
         R.w4a16(t(A2, dev), i32(inputs.pack_w4(q2), dev), t(s2, dev), i32(inputs.pack_zeros(z2), dev), 16,
                 config=KIND5)
+
+
+KIND6 = W4A16Config(6, 128, 128, 1)
+
+
+@pytest.mark.parametrize("M", [1, 65, 128, 130, 256, 300])
+@pytest.mark.parametrize("group", [64, 128, 256])
+@pytest.mark.parametrize("N,K", [(544, 1024), (2080, 4096)])
+def test_w4a16_streamk(dev, orc, M, group, N, K):
+    # N % 128 == 32: a ragged last column tile; K = 4096 at N = 2080 gives
+    # ranges that start and end inside tiles and tiles shared by 3-4 blocks
+    A, q, s, z = inputs.w4a16_problem(M, N, K, group, seed=211 + M + N)
+    packed, qz = inputs.pack_w4(q), inputs.pack_zeros(z)
+    C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), group, config=KIND6).cpu().numpy()
+    w4_close(C, orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, group)))
+
+
+@pytest.mark.parametrize("K", [512, 1408, 2048])
+def test_w4a16_streamk_exact_and_symmetric(dev, orc, K):
+    # scale 1, A small integers: every FP32 sum is an integer below 2^16, so the
+    # split tiles' partials add up exactly and the result equals the oracle bit
+    # for bit.  K = 512: tiles shared by two blocks; 1408: block ranges that
+    # cross tile edges; 2048: tiles shared by eight blocks
+    M, N = 200, 288
+    rng = np.random.default_rng(K)
+    q = rng.integers(0, 16, (K, N)).astype(np.uint8)
+    z = rng.integers(0, 16, (K // 128, N)).astype(np.uint8)
+    s = np.ones((K // 128, N), np.float16)
+    A = rng.integers(-2, 3, (M, K)).astype(np.float16)
+    packed, qz = orc.w4_pack(q), orc.w4_pack_zeros(z)
+    for zeros in (qz, None):
+        want = orc.w4a16_rows(A, orc.w4_dequant(packed, s, zeros, 128))
+        C = R.w4a16(t(A, dev), i32(packed, dev), t(s, dev), None if zeros is None else i32(zeros, dev), 128,
+                    config=KIND6).cpu().numpy()
+        assert np.array_equal(C.view(np.uint16), want.view(np.uint16)), (K, zeros is None)
+
+
+def test_w4a16_streamk_repeatable(dev, orc):
+    # the arrival counters return to zero after each call: repeated calls,
+    # and a call of another shape in between, give identical bits
+    shapes = [(128, 2080, 4096), (200, 544, 1408)]
+    data = []
+    for M, N, K in shapes:
+        A, q, s, z = inputs.w4a16_problem(M, N, K, 128, seed=M + K)
+        data.append((t(A, dev), i32(inputs.pack_w4(q), dev), t(s, dev), i32(inputs.pack_zeros(z), dev)))
+    first = [R.w4a16(*d, 128, config=KIND6).clone() for d in data]
+    for _ in range(3):
+        for d, f in zip(data, first):
+            assert torch.equal(R.w4a16(*d, 128, config=KIND6), f)
+
+
+def test_w4a16_streamk_rejects_and_auto_falls_back(dev, orc):
+    from libthom_amd._lib import LthipError
+    A, q, s, z = inputs.w4a16_problem(128, 520, 512, 128, seed=31)   # N % 32 != 0
+    args = (t(A, dev), i32(inputs.pack_w4(q), dev), t(s, dev), i32(inputs.pack_zeros(z), dev), 128)
+    with pytest.raises(LthipError):
+        R.w4a16(*args, config=KIND6)
+    A2, q2, s2, z2 = inputs.w4a16_problem(128, 544, 512, 32, seed=32)   # group 32
+    packed2, qz2 = inputs.pack_w4(q2), inputs.pack_zeros(z2)
+    args2 = (t(A2, dev), i32(packed2, dev), t(s2, dev), i32(qz2, dev), 32)
+    for cfg in (KIND6, W4A16Config(6, 128, 128, 2), W4A16Config(6, 256, 128, 1)):
+        with pytest.raises(LthipError):
+            R.w4a16(*args2, config=cfg)
+    # the automatic choice at M = 128 is kind 6; where kind 6 does not take
+    # the shape it falls back (group 32: kind 2), within the product tolerance
+    w4_close(R.w4a16(*args2).cpu().numpy(), orc.w4a16_rows(A2, orc.w4_dequant(packed2, s2, qz2, 32)))

@@ -24,6 +24,7 @@ CASES = (
     [(W4A16Config(1, 64, 16, sp), M, 32) for sp in (1, 2) for M in (1, 16, 100)] +
     [(W4A16Config(2, bn, bm, 1), M, 32) for bm, bn in ((64, 64), (128, 128), (256, 256)) for M in (1, 70, 300)] +
     [(W4A16Config(3, bn, bm, 1), M, 64) for bm, bn in ((128, 128), (256, 256), (64, 256), (64, 128), (64, 64)) for M in (1, 70, 300)] +
+    [(W4A16Config(6, 128, 128, 1), M, 64) for M in (1, 70, 128, 300)] +
     [(None, M, 32) for M in (1, 2, 3, 16, 64, 65, 200)] + [(None, M, 64) for M in (1, 16, 128, 300)]
 )
 

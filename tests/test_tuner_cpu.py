@@ -24,7 +24,9 @@ def test_db_roundtrip(tmp_path):
 
 
 def test_space_and_validity():
-    assert len(tuner.W4A16_SPACE) == 34
+    assert len(tuner.W4A16_SPACE) == 35
+    assert tuner.valid((6, 128, 128, 1), 128, 4096, 11008, 128) and not tuner.valid((6, 128, 128, 1), 128, 4096, 520, 128)
+    assert not tuner.valid((6, 128, 128, 1), 128, 4096, 11008, 32) and not tuner.valid((6, 128, 128, 1), 128, 4096)
     assert tuner.valid((4, 64, 0, 1), 4, 4096, 11008, 32) and not tuner.valid((4, 64, 0, 1), 5, 4096, 11008, 128)
     assert tuner.valid((4, 32, 0, 1), 2, 4096, 4096, 64) and not tuner.valid((4, 32, 0, 1), 2, 4096, 4096, 32)
     assert not tuner.valid((4, 64, 0, 1), 4, 16384, 4096, 128)
