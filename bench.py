@@ -624,9 +624,9 @@ def w4a16_kind_name(M, N, K):
         return "skinny MFMA (kind 5)"
     if N % 32 != 0:
         return "tiled MFMA (kind 2)"
-    if M <= 256:
+    if M <= 256 and ((M + 127) // 128) * ((N + 127) // 128) * (K // 64) >= 8 * 256:
         return "stream-K pipelined tile (kind 6, 128x128)"
-    return "pipelined MFMA tile (kind 3, 128x128 or 256x256)"
+    return "pipelined MFMA tile (kind 3, " + ("64x128)" if M <= 256 else "128x128 or 256x256)")
 
 
 def add_measured_fractions(routines, peaks):

@@ -902,12 +902,16 @@ void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_
             c.tile_n = 64;
             c.tile_m = 0;
             c.split_k = 1;
-        } else if (N % 32 == 0 && M <= 256) {
+        } else if (N % 32 == 0 && M <= 256 &&
+                   ((M + 127) / 128) * ((N + 127) / 128) * (K / 64) >= 8 * (size_t)cu_count()) {
             // stream-K over 128 x 128 tiles whose waves hold all rows (each
             // weight dequantised once, every CU busy): at N = 11008, K = 4096,
             // M = 128 in 27 against 32 us for the 64 x 128 pipelined tile,
-            // M = 256 in 40 against 46 us; the launch falls back to that tile
-            // where kind 6 does not take the shape
+            // M = 256 in 40 against 46 us.  Over the tuner's sweep it beats
+            // that tile on 9 shapes in 10 where each block gets at least 8
+            // tile-slabs, on 6 in 10 below (tiles shared by more blocks, more
+            // partials to collect): those keep the 64 x 128 tile.  The launch
+            // falls back to that tile where kind 6 does not take the shape
             c.kind = 6;
             c.tile_m = 128;
             c.tile_n = 128;
