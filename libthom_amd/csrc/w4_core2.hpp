@@ -130,6 +130,72 @@ struct W4Src {
     }
 };
 
+// W4Src for the look-ahead tiles: A and W through buffer loads to LDS, the
+// per-lane offsets fixed at init and the slab's offset a scalar (soffset), so
+// advancing a slab costs no vector instructions.  Needs A's rows of the tile
+// and W below 2^31 bytes (checked by the launch).
+template <class C>
+struct W4SrcB {
+    __amdgpu_buffer_rsrc_t ra, rw;
+    int va[C::A_INSTR], vw[C::B_INSTR];
+    int w_slab_bytes;
+    W4Src<C> base;   // the scale / zero-point source (one pointer load a slab)
+
+    __device__ __forceinline__ void init(const __half* __restrict__ A, long lda, const BW4Params& p, int M, int N,
+                                         int m0, int n0, int k0) {
+        const int lane = threadIdx.x & 63;
+        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        base.init(A, lda, p, M, N, m0, n0, k0);
+        ra = __builtin_amdgcn_make_buffer_rsrc((void*)(A + (long)m0 * lda + k0), 0, 0x7FFFFFFF, 0x00020000);
+        rw = __builtin_amdgcn_make_buffer_rsrc((void*)(p.W + (long)(k0 / 8) * N + n0), 0, 0x7FFFFFFF, 0x00020000);
+#pragma unroll
+        for (int i = 0; i < C::A_INSTR; i++) {
+            const int inst = wave * C::A_INSTR + i;
+            const int row = inst * C::A_ROWS_PER + lane / C::KC;
+            const int kc = (lane % C::KC) ^ swzA2<C::BK>(row);
+            int gm = m0 + row;
+            gm = gm < M ? gm : M - 1;
+            va[i] = (int)(((long)(gm - m0) * lda + kc * 8) * 2);
+        }
+#pragma unroll
+        for (int i = 0; i < C::B_INSTR; i++) {
+            const int inst = wave * C::B_INSTR + i;
+            const int row = inst * C::B_ROWS_PER + lane / C::NC;
+            const int ch = (lane % C::NC) ^ swzW2(row);
+            int gn = n0 + ch * 4;
+            gn = gn < N ? gn : N - 4;
+            vw[i] = (int)(((long)row * N + (gn - n0)) * 4);
+        }
+        w_slab_bytes = (C::BK / 8) * N * 4;
+    }
+
+    template <int Q0, int Q1>
+    __device__ __forceinline__ void issue(char* smem, int stage, int slab) const {
+        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        char* sa = smem + stage * C::STAGE_BYTES;
+        char* sb = sa + C::A_BYTES;
+        char* ss = sb + C::B_BYTES;
+#pragma unroll
+        for (int q = Q0; q < Q1; q++) {
+            if (q < C::A_INSTR) {
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    ra, (lds_void_t*)(lds_char_t*)(sa + (wave * C::A_INSTR + q) * 1024), 16, va[q], slab * C::BK * 2,
+                    0, 0);
+            } else if (q < C::A_INSTR + C::B_INSTR) {
+                const int i = q - C::A_INSTR;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rw, (lds_void_t*)(lds_char_t*)(sb + (wave * C::B_INSTR + i) * 1024), 16, vw[i],
+                    slab * w_slab_bytes, 0, 0);
+            } else {
+                const int cg = wave % C::WGN;
+                __builtin_amdgcn_global_load_lds(base.sz + (long)((base.slab0 + slab) / base.group_slabs) *
+                                                               base.sz_group_step,
+                                                 (lds_void_t*)(lds_char_t*)(ss + cg * C::SZ_STRIDE), 16, 0, 0);
+            }
+        }
+    }
+};
+
 template <class C>
 struct W4Frags {
     v8h a[C::TM];
@@ -295,7 +361,7 @@ __device__ __forceinline__ void w4_mainloop_la(char* smem, const __half* __restr
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int cg = wave % C::WGN;
     const int wm = (wave / C::WGN) * C::WM, wn = cg * C::WN;
-    W4Src<C> src;
+    W4SrcB<C> src;
     src.init(A, lda, p, M, N, m0, n0, s0 * BK);
     wait_vmcnt<0>();
 #pragma unroll

@@ -1091,11 +1091,12 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
         const bool ok = e && group % 64 == 0 && N % 32 == 0 && K > 0 &&
                         reinterpret_cast<uintptr_t>(scales) % 16 == 0 &&
                         reinterpret_cast<uintptr_t>(qzeros) % 16 == 0 &&
-                        ((M + e->bm - 1) / e->bm) * ((N + e->bn - 1) / e->bn) <= kMaxCounters;
+                        ((M + e->bm - 1) / e->bm) * ((N + e->bn - 1) / e->bn) <= kMaxCounters &&
+                        M * K * 2 < (1u << 31) && (K / 8) * N * 4 < (1u << 31);   // 32-bit buffer offsets
         if (!ok && explicit6)
             return fail(LTHIP_ERR_UNSUPPORTED,
                         "w4a16: kind 6 needs a built tile (128x128), K > 0, group %% 64 == 0, N %% 32 == 0, "
-                        "16-byte aligned scales/zero-points and at most %zu tiles",
+                        "16-byte aligned scales/zero-points, at most %zu tiles, A and W below 2 GiB",
                         kMaxCounters);
         if (explicit6 && c.split_k != 1) return fail(LTHIP_ERR_INVALID_ARG, "w4a16: kind 6 takes split_k = 1");
         if (ok) {
