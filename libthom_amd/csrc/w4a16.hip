@@ -705,7 +705,7 @@ constexpr int kNumTiled = sizeof(kTiled) / sizeof(kTiled[0]);
 // tail split, block full + j runs split j % tail of tile full + j / tail, so
 // the tiles left over after the last full round of launch slots are shared
 // by all slots instead of leaving most of them idle.
-template <class C>
+template <class C, bool LA>
 __global__ __launch_bounds__(C::THREADS) void w4a16_tiled2_kernel(const __half* __restrict__ A, BW4Params p,
                                                                    __half* __restrict__ Cm, int M, int N, int K,
                                                                    int tiles_m, int full, int tail,
@@ -733,19 +733,23 @@ __global__ __launch_bounds__(C::THREADS) void w4a16_tiled2_kernel(const __half* 
     o.counter = counters ? counters + t : nullptr;
     const int slabs = K / C::BK;
     const int s0 = (int)((long)slabs * split / splits), s1 = (int)((long)slabs * (split + 1) / splits);
-    w4_tile2<C>(smem, A, (long)K, p, M, N, tm * C::BM, tn * C::BN, s0, s1, split, splits, o);
+    w4_tile2<C, LA>(smem, A, (long)K, p, M, N, tm * C::BM, tn * C::BN, s0, s1, split, splits, o);
 }
 
 typedef void (*tiled2_fn)(const __half*, BW4Params, __half*, int, int, int, int, int, int, float*, unsigned*);
+// fn_la: the look-ahead loop with buffer-load DMA (A and W below 2^31 bytes)
 struct Tiled2Entry {
     int bm, bn, smem;
-    tiled2_fn fn;
+    tiled2_fn fn, fn_la;
     int threads;
 };
-#define LT_W4T2(bm, bn, st, wm, wn)                                                                    \
-    {bm, bn, Core2W4<bm, bn, st, wm, wn>::SMEM, w4a16_tiled2_kernel<Core2W4<bm, bn, st, wm, wn>>, \
-     Core2W4<bm, bn, st, wm, wn>::THREADS},
-const Tiled2Entry kTiled2[] = {LT_W4T2(256, 256, 3, 2, 4) LT_W4T2(256, 128, 3, 2, 2) LT_W4T2(128, 256, 3, 2, 4)
+#define LT_W4T2(bm, bn, st, wm, wn)                                                                           \
+    {bm, bn, Core2W4<bm, bn, st, wm, wn>::SMEM, w4a16_tiled2_kernel<Core2W4<bm, bn, st, wm, wn>, false>,   \
+     w4a16_tiled2_kernel<Core2W4<bm, bn, st, wm, wn>, true>, Core2W4<bm, bn, st, wm, wn>::THREADS},
+// 256 x 256: eight waves of 256 x 32 (each weight word dequantised once per
+// block and fed to eight MFMAs; with the look-ahead loop M = 2048 at N =
+// 11008, K = 4096 in 193 against 206 us for 2 x 4 waves of 128 x 64)
+const Tiled2Entry kTiled2[] = {LT_W4T2(256, 256, 3, 1, 8) LT_W4T2(256, 128, 3, 2, 2) LT_W4T2(128, 256, 3, 2, 4)
                                    LT_W4T2(128, 128, 3, 2, 2) LT_W4T2(64, 256, 3, 1, 8) LT_W4T2(64, 128, 3, 1, 4)
                                        LT_W4T2(64, 64, 3, 1, 2)};
 #undef LT_W4T2
@@ -1165,7 +1169,8 @@ int lthip_w4a16_gemm(const uint16_t* A, const uint32_t* W_packed, const uint16_t
                 partial = reinterpret_cast<float*>(reinterpret_cast<char*>(ws) + kCounterBytes);
             }
             const unsigned blocks = (unsigned)(full + (tiles - full) * tail);
-            hipLaunchKernelGGL(e.fn, dim3(blocks, (unsigned)splits), dim3((unsigned)e.threads), 0, s,
+            const bool la = M * K * 2 < (1u << 31) && (K / 8) * N * 4 < (1u << 31);   // 32-bit buffer offsets
+            hipLaunchKernelGGL(la ? e.fn_la : e.fn, dim3(blocks, (unsigned)splits), dim3((unsigned)e.threads), 0, s,
                                reinterpret_cast<const __half*>(A), p, reinterpret_cast<__half*>(C), m, n, k, tiles_m,
                                full, tail, partial, counters);
             LT_TRY_LAUNCH("w4a16 tiled2 launch");

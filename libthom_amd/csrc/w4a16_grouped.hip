@@ -26,7 +26,8 @@
 namespace lt {
 namespace {
 
-using GW4Big = Core2W4<256, 256, 3, 2, 4>;
+// the big tile as in the single launch: eight waves of 256 x 32
+using GW4Big = Core2W4<256, 256, 3, 1, 8>;
 using GW4Mid = Core2W4<128, 256, 3, 2, 4>;
 using GW4Small = Core2W4<64, 256, 3, 1, 8>;
 static_assert(GW4Big::THREADS == GW4Mid::THREADS && GW4Mid::THREADS == GW4Small::THREADS, "one block shape");
@@ -43,6 +44,9 @@ struct GW4Prob {
     int M, N, K, cls, tiles_m, work_begin;
 };
 
+// LA: the look-ahead tile loop with buffer-load DMA (every problem's A and W
+// below 2^31 bytes)
+template <bool LA>
 __global__ __launch_bounds__(kGW4Threads) void w4a16_grouped_kernel(const GW4Prob* __restrict__ probs, int count,
                                                                     int total, unsigned* __restrict__ queue) {
     __shared__ __align__(16) char smem[kGW4Smem];
@@ -70,11 +74,11 @@ __global__ __launch_bounds__(kGW4Threads) void w4a16_grouped_kernel(const GW4Pro
         o.counter = nullptr;
         const int slabs = q.K / 64;
         if (q.cls == 0)
-            w4_tile2<GW4Big>(smem, q.A, (long)q.K, q.p, q.M, q.N, tm * 256, tn * kGW4TileN, 0, slabs, 0, 1, o);
+            w4_tile2<GW4Big, LA>(smem, q.A, (long)q.K, q.p, q.M, q.N, tm * 256, tn * kGW4TileN, 0, slabs, 0, 1, o);
         else if (q.cls == 1)
-            w4_tile2<GW4Mid>(smem, q.A, (long)q.K, q.p, q.M, q.N, tm * 128, tn * kGW4TileN, 0, slabs, 0, 1, o);
+            w4_tile2<GW4Mid, LA>(smem, q.A, (long)q.K, q.p, q.M, q.N, tm * 128, tn * kGW4TileN, 0, slabs, 0, 1, o);
         else
-            w4_tile2<GW4Small>(smem, q.A, (long)q.K, q.p, q.M, q.N, tm * 64, tn * kGW4TileN, 0, slabs, 0, 1, o);
+            w4_tile2<GW4Small, LA>(smem, q.A, (long)q.K, q.p, q.M, q.N, tm * 64, tn * kGW4TileN, 0, slabs, 0, 1, o);
         __syncthreads();
     }
     // the last block out resets the queue for the next launch / graph replay
@@ -126,7 +130,7 @@ int gw4_check(const lthip_w4a16_problem& q, int i) {
 
 // problem table in launch order (longest work item first) and the workspace size
 int gw4_layout(const lthip_w4a16_problem* probs, int count, const lthip_w4a16_config* cfg,
-               std::vector<GW4Prob>* table, size_t* ws_total, int* total) {
+               std::vector<GW4Prob>* table, size_t* ws_total, int* total, bool* la = nullptr) {
     std::vector<int> cls(count, 0), order;
     for (int i = 0; i < count; i++) {
         int st = gw4_check(probs[i], i);
@@ -140,6 +144,7 @@ int gw4_layout(const lthip_w4a16_problem* probs, int count, const lthip_w4a16_co
     });
     table->clear();
     long work = 0;
+    if (la) *la = true;
     for (int i : order) {
         const lthip_w4a16_problem& q = probs[i];
         GW4Prob g;
@@ -152,6 +157,7 @@ int gw4_layout(const lthip_w4a16_problem* probs, int count, const lthip_w4a16_co
         g.cls = cls[i];
         g.tiles_m = (q.M + kGW4TileM[g.cls] - 1) / kGW4TileM[g.cls];
         g.work_begin = (int)work;
+        if (la && ((size_t)q.M * q.K * 2 >= (1u << 31) || (size_t)(q.K / 8) * q.N * 4 >= (1u << 31))) *la = false;
         work += (long)g.tiles_m * ((q.N + kGW4TileN - 1) / kGW4TileN);
         if (work > 0x7fffffffL) return fail(LTHIP_ERR_INVALID_ARG, "grouped w4a16: too many work items");
         table->push_back(g);
@@ -168,6 +174,7 @@ struct lthip_w4a16_grouped_plan {
     int count;   // problems with work
     int total;   // work items
     int grid;
+    bool la;              // every problem fits the look-ahead loop's 32-bit offsets
     lt::GW4Prob* table;   // device (inside the workspace)
     unsigned* queue;      // device, 2 counters
     hipGraph_t graph;
@@ -195,7 +202,8 @@ int lthip_w4a16_grouped_plan_create(const lthip_w4a16_problem* probs, int count,
     std::vector<GW4Prob> table;
     size_t need = 0;
     int work = 0;
-    int st = gw4_layout(probs, count, cfg, &table, &need, &work);
+    bool la = true;
+    int st = gw4_layout(probs, count, cfg, &table, &need, &work, &la);
     if (st != LTHIP_OK) return st;
     if (!ws || ws_bytes < need) return fail(LTHIP_ERR_WORKSPACE, "grouped w4a16: need %zu workspace bytes", need);
     if (reinterpret_cast<uintptr_t>(ws) % 256) return fail(LTHIP_ERR_INVALID_ARG, "grouped w4a16: workspace alignment");
@@ -216,6 +224,7 @@ int lthip_w4a16_grouped_plan_create(const lthip_w4a16_problem* probs, int count,
     const int per_cu = 163840 / (kGW4Smem + 64);
     const int cap = cu_count() * (per_cu >= 1 ? per_cu : 1);
     p->grid = work < cap ? (work > 0 ? work : 1) : cap;
+    p->la = la;
     p->table = dtable;
     p->queue = reinterpret_cast<unsigned*>(base);
     p->graph = nullptr;
@@ -228,8 +237,9 @@ int lthip_w4a16_grouped_run(lthip_w4a16_grouped_plan* plan, void* stream) {
     using namespace lt;
     if (!plan) return fail(LTHIP_ERR_INVALID_ARG, "grouped w4a16: null plan");
     if (plan->total == 0) return LTHIP_OK;
-    hipLaunchKernelGGL(w4a16_grouped_kernel, dim3((unsigned)plan->grid), dim3(kGW4Threads), 0, as_stream(stream),
-                       plan->table, plan->count, plan->total, plan->queue);
+    hipLaunchKernelGGL(plan->la ? w4a16_grouped_kernel<true> : w4a16_grouped_kernel<false>,
+                       dim3((unsigned)plan->grid), dim3(kGW4Threads), 0, as_stream(stream), plan->table, plan->count,
+                       plan->total, plan->queue);
     LT_TRY_LAUNCH("grouped w4a16 launch");
     return LTHIP_OK;
 }
@@ -250,8 +260,9 @@ int lthip_w4a16_grouped_capture(lthip_w4a16_grouped_plan* plan, void* stream) {
     }
     int st = check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "grouped w4a16: begin capture");
     if (st != LTHIP_OK) return st;
-    hipLaunchKernelGGL(w4a16_grouped_kernel, dim3((unsigned)plan->grid), dim3(kGW4Threads), 0, s, plan->table,
-                       plan->count, plan->total, plan->queue);
+    hipLaunchKernelGGL(plan->la ? w4a16_grouped_kernel<true> : w4a16_grouped_kernel<false>,
+                       dim3((unsigned)plan->grid), dim3(kGW4Threads), 0, s, plan->table, plan->count, plan->total,
+                       plan->queue);
     hipError_t le = hipGetLastError();
     hipGraph_t g = nullptr;
     st = check_hip(hipStreamEndCapture(s, &g), "grouped w4a16: end capture");
