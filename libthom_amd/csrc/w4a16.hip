@@ -746,11 +746,16 @@ struct Tiled2Entry {
 #define LT_W4T2(bm, bn, st, wm, wn)                                                                           \
     {bm, bn, Core2W4<bm, bn, st, wm, wn>::SMEM, w4a16_tiled2_kernel<Core2W4<bm, bn, st, wm, wn>, false>,   \
      w4a16_tiled2_kernel<Core2W4<bm, bn, st, wm, wn>, true>, Core2W4<bm, bn, st, wm, wn>::THREADS},
-// 256 x 256: eight waves of 256 x 32 (each weight word dequantised once per
-// block and fed to eight MFMAs; with the look-ahead loop M = 2048 at N =
-// 11008, K = 4096 in 193 against 206 us for 2 x 4 waves of 128 x 64)
-const Tiled2Entry kTiled2[] = {LT_W4T2(256, 256, 3, 1, 8) LT_W4T2(256, 128, 3, 2, 2) LT_W4T2(128, 256, 3, 2, 4)
-                                   LT_W4T2(128, 128, 3, 2, 2) LT_W4T2(64, 256, 3, 1, 8) LT_W4T2(64, 128, 3, 1, 4)
+// Every tile's waves hold all of its rows (one row of waves): each weight
+// word is dequantised once per block and feeds BM / 32 MFMAs.  At N = 11008,
+// K = 4096 with the look-ahead loop: 256 x 256 (eight waves of 256 x 32) at
+// M = 2048 in 193 against 206 us for 2 x 4 waves of 128 x 64; 128 x 128
+// (four of 128 x 32) at M = 300 / 512 / 1024 in 57 / 63 / 119 against
+// 67 / 73 / 131 us for 2 x 2 waves of 64 x 64; 256 x 128 (four of 256 x 32)
+// 55 / 59 / 114 against 69 / 69 / 127 us; 128 x 256 as eight waves of
+// 128 x 32 level with 2 x 4 waves of 64 x 64
+const Tiled2Entry kTiled2[] = {LT_W4T2(256, 256, 3, 1, 8) LT_W4T2(256, 128, 3, 1, 4) LT_W4T2(128, 256, 3, 1, 8)
+                                   LT_W4T2(128, 128, 3, 1, 4) LT_W4T2(64, 256, 3, 1, 8) LT_W4T2(64, 128, 3, 1, 4)
                                        LT_W4T2(64, 64, 3, 1, 2)};
 #undef LT_W4T2
 constexpr int kNumTiled2 = sizeof(kTiled2) / sizeof(kTiled2[0]);
