@@ -926,12 +926,10 @@ void resolve(size_t M, size_t N, size_t K, const lthip_w4a16_config* cfg, lthip_
             c.tile_n = 128;
             c.split_k = 1;
         } else if (N % 32 == 0) {
-            // pipelined tiles: 64 x 128 up to M = 256 (at N = 11008, K = 4096:
-            // M = 128 in 33 against 43 us for 128 x 128 with a K-split tail,
-            // M = 256 in 49 against 56 us), then 128 x 128, or 256 x 256
+            // pipelined tiles: 64 x 128 up to M = 256 where kind 6 gives a
+            // block fewer than 8 tile-slabs, then 128 x 128, or 256 x 256
             // (8 waves) once those cover half the CUs (256 x 256 faster from
-            // M = 1024; the 128 x 256 tiles were slower than either at M =
-            // 256-512); the launch shares the last partial round of tiles
+            // M = 1024); the launch shares the last partial round of tiles
             // out by splitting K
             c.kind = 3;
             const long tiles256 = (long)((M + 255) / 256) * (long)((N + 255) / 256);
