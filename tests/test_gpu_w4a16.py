@@ -354,6 +354,27 @@ def test_w4a16_streamk_repeatable(dev, orc):
             assert torch.equal(R.w4a16(*d, 128, config=KIND6), f)
 
 
+@pytest.mark.parametrize("M,N,K,group", [(65, 512, 2048, 32), (261, 512, 3072, 32), (130, 1856, 2048, 16),
+                                          (300, 704, 1024, 32), (1100, 640, 1024, 32), (96, 3520, 1536, 8)])
+def test_w4a16_auto_kind2_fallback(dev, orc, M, N, K, group):
+    # group not a multiple of 64, M > 64: the automatic choice runs the
+    # LDS-dequantising tile with its own tile and K split (split 8 at
+    # 261 x 512 x 3072); with the queried workspace and with none (the split
+    # shrinks to 1) both meet the product tolerance
+    from libthom_amd._lib import call, lib
+    A, q, s, z = inputs.w4a16_problem(M, N, K, group, seed=M + N)
+    packed, qz = inputs.pack_w4(q), inputs.pack_zeros(z)
+    ref = orc.w4a16_rows(A, orc.w4_dequant(packed, s, qz, group))
+    args = (t(A, dev), i32(packed, dev), t(s, dev), i32(qz, dev), group)
+    if (M, N, K) == (261, 512, 3072):
+        assert lib().lthip_w4a16_workspace_size(M, N, K, None) >= 8 * M * N * 4
+    w4_close(R.w4a16(*args).cpu().numpy(), ref)
+    C = torch.empty(M, N, dtype=torch.float16, device=dev)
+    call("lthip_w4a16_gemm", args[0].data_ptr(), args[1].data_ptr(), args[2].data_ptr(), args[3].data_ptr(),
+         C.data_ptr(), M, N, K, group, None, None, 0, torch.cuda.current_stream(dev).cuda_stream)
+    w4_close(C.cpu().numpy(), ref)
+
+
 def test_w4a16_streamk_rejects_and_auto_falls_back(dev, orc):
     from libthom_amd._lib import LthipError
     A, q, s, z = inputs.w4a16_problem(128, 520, 512, 128, seed=31)   # N % 32 != 0
