@@ -219,6 +219,10 @@ typedef struct lthip_w4a16_config {
     int split_k;     /* K splits (>= 1); > 1 needs the workspace; kinds 4, 5, 6: 1 */
 } lthip_w4a16_config;
 
+/* With cfg NULL or kind 0 the size also covers the fallbacks the launch takes
+ * on what the query does not see (group, alignment): the 64 x 128 pipelined
+ * tile's tail split and kind 2's K split.  Given less, those launches split
+ * K less (other summation order, same tolerance). */
 size_t lthip_w4a16_workspace_size(size_t M, size_t N, size_t K, const lthip_w4a16_config *cfg);
 int lthip_w4a16_gemm(const uint16_t *A, const uint32_t *W_packed, const uint16_t *scales,
                      const uint32_t *qzeros, uint16_t *C, size_t M, size_t N, size_t K, int group,
